@@ -1,0 +1,394 @@
+"""Oracle for the Mille-feuille hot path -- TEST INFRASTRUCTURE ONLY.
+
+Python access to the C restatement in ``mf_oracle.c`` (built as ``liboracle.so``)
+plus NumPy restatements used as checkers.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import
+this module, and only to check or to time the reference algorithm on the CPU.
+The product package never imports it.
+
+What each piece restates and how it is pinned is listed in ``mf_oracle.h``.
+NumPy-only pieces:
+
+* ``mixed_tile_spmv`` -- the build's tile-mixed SpMV semantics (values of a
+  tile read at its precision class, the gathered vector converted to binary32,
+  products formed in binary32, summed in binary64), restating the per-tile
+  ``switch(d_vis[colid])`` of main-cg-mixed.cu:1130-1214 with binary32 products
+  instead of the reference's half/int8 accumulators.  Pinned by exact-integer
+  cases (all products exact) and by its FP64 class reducing to ``tile_spmv``.
+* ``static_class_np`` -- vectorised restatement of ``oracle_static_class`` used to
+  cross-check the C one on many values.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+_LIB = None
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class _Tiles(ctypes.Structure):
+    _fields_ = [
+        ("tilem", ctypes.c_int), ("tilen", ctypes.c_int), ("tilenum", ctypes.c_int),
+        ("csrsize", ctypes.c_int), ("csrptrlen", ctypes.c_int),
+        ("tile_ptr", _i32p), ("tile_columnidx", _i32p), ("tile_nnz", _i32p),
+        ("Format", ctypes.POINTER(ctypes.c_byte)), ("blknnz", _i32p),
+        ("blknnznnz", _u8p), ("tilewidth", ctypes.POINTER(ctypes.c_byte)),
+        ("csr_offset", _i32p), ("csrptr_offset", _i32p),
+        ("Blockcsr_Val", _f64p), ("Blockcsr_Val_Low", _f32p),
+        ("Tile_csr_Col", _u8p), ("Blockcsr_Ptr", _u8p), ("csr_compressedIdx", _u8p),
+    ]
+
+
+def build() -> pathlib.Path:
+    """Compile liboracle.so (gcc) if missing or stale."""
+    so = HERE / "liboracle.so"
+    src = [HERE / "mf_oracle.c", HERE / "mf_oracle.h"]
+    if not so.exists() or any(s.stat().st_mtime > so.stat().st_mtime for s in src):
+        subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = ctypes.CDLL(str(build()))
+        L = _LIB
+        L.oracle_tile_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                                         _f64p, _f32p, ctypes.POINTER(_Tiles)]
+        L.oracle_tile_free.argtypes = [ctypes.POINTER(_Tiles)]
+        L.oracle_prep_tables.argtypes = [ctypes.POINTER(_Tiles), ctypes.c_int, _i32p, _i32p, _i32p,
+                                         ctypes.POINTER(ctypes.c_uint), _i32p, _i32p]
+        L.oracle_row_lists.argtypes = [ctypes.POINTER(_Tiles), ctypes.c_int, _i32p, _u8p, _u8p, _i32p]
+        L.oracle_encode.argtypes = [_u8p, _u8p, ctypes.c_int, ctypes.c_int]
+        L.oracle_decode.argtypes = [_u8p, _u8p, ctypes.c_int]
+        L.oracle_tile_spmv.argtypes = [ctypes.POINTER(_Tiles), ctypes.c_int, _f64p, _f64p]
+        L.oracle_csr_spmv.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        for name in ("oracle_cg", "oracle_bicgstab"):
+            getattr(L, name).argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                         ctypes.c_double, ctypes.c_int, _i32p, _f64p, _f64p]
+            getattr(L, name).restype = ctypes.c_int
+        L.oracle_dyn_code.argtypes = [ctypes.c_double]
+        L.oracle_dyn_class.argtypes = [_f64p, _f64p, ctypes.c_int]
+        L.oracle_static_class.argtypes = [_f64p, ctypes.c_int]
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _arr(ptr, n, dtype):
+    if n <= 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+TILE_FIELDS = ("tile_ptr", "tile_columnidx", "tile_nnz", "Format", "blknnz", "blknnznnz",
+               "tilewidth", "csr_offset", "csrptr_offset", "Blockcsr_Val", "Blockcsr_Val_Low",
+               "Tile_csr_Col", "Blockcsr_Ptr", "csr_compressedIdx")
+
+
+def tile_create(rowA, colA, rowptr, colidx, val, val_low=None):
+    """Tile_create restatement; returns a dict of NumPy arrays plus scalars."""
+    L = lib()
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    if val_low is None:
+        val_low = val.astype(np.float32)
+    val_low = np.ascontiguousarray(val_low, dtype=np.float32)
+    t = _Tiles()
+    L.oracle_tile_create(int(rowA), int(colA), int(colidx.size), _p(rowptr, _i32p), _p(colidx, _i32p),
+                         _p(val, _f64p), _p(val_low, _f32p), ctypes.byref(t))
+    T = t.tilenum
+    out = dict(tilem=t.tilem, tilen=t.tilen, tilenum=T, csrsize=t.csrsize, csrptrlen=t.csrptrlen)
+    csize = t.csrsize // 2 if t.csrsize % 2 == 0 else t.csrsize // 2 + 1
+    sizes = dict(tile_ptr=t.tilem + 1, tile_columnidx=T, tile_nnz=T + 1, Format=T, blknnz=T + 1,
+                 blknnznnz=T + 1, tilewidth=T, csr_offset=T + 1, csrptr_offset=T + 1,
+                 Blockcsr_Val=t.csrsize, Blockcsr_Val_Low=t.csrsize, Tile_csr_Col=t.csrsize,
+                 Blockcsr_Ptr=t.csrptrlen, csr_compressedIdx=csize)
+    dt = dict(tile_ptr=np.int32, tile_columnidx=np.int32, tile_nnz=np.int32, Format=np.int8,
+              blknnz=np.int32, blknnznnz=np.uint8, tilewidth=np.int8, csr_offset=np.int32,
+              csrptr_offset=np.int32, Blockcsr_Val=np.float64, Blockcsr_Val_Low=np.float32,
+              Tile_csr_Col=np.uint8, Blockcsr_Ptr=np.uint8, csr_compressedIdx=np.uint8)
+    for f in TILE_FIELDS:
+        out[f] = _arr(getattr(t, f), sizes[f], dt[f])
+    out["_rowA"] = int(rowA)
+    out["_colA"] = int(colA)
+    L.oracle_tile_free(ctypes.byref(t))
+    return out
+
+
+def _to_struct(d):
+    """Rebuild the C struct view (arrays kept alive in the returned tuple)."""
+    keep = {}
+    t = _Tiles()
+    for k in ("tilem", "tilen", "tilenum", "csrsize", "csrptrlen"):
+        setattr(t, k, int(d[k]))
+    ptypes = dict(tile_ptr=_i32p, tile_columnidx=_i32p, tile_nnz=_i32p,
+                  Format=ctypes.POINTER(ctypes.c_byte), blknnz=_i32p, blknnznnz=_u8p,
+                  tilewidth=ctypes.POINTER(ctypes.c_byte), csr_offset=_i32p, csrptr_offset=_i32p,
+                  Blockcsr_Val=_f64p, Blockcsr_Val_Low=_f32p, Tile_csr_Col=_u8p, Blockcsr_Ptr=_u8p,
+                  csr_compressedIdx=_u8p)
+    for f in TILE_FIELDS:
+        a = np.ascontiguousarray(d[f])
+        if a.size == 0:
+            a = np.zeros(1, dtype=a.dtype)
+        keep[f] = a
+        setattr(t, f, a.ctypes.data_as(ptypes[f]))
+    return t, keep
+
+
+def prep_tables(d):
+    L = lib()
+    t, keep = _to_struct(d)
+    T = int(d["tilenum"])
+    p1 = np.zeros(max(T, 1), np.int32)
+    p2 = np.zeros(max(T, 1), np.int32)
+    rbb = ctypes.c_int(0)
+    L.oracle_prep_tables(ctypes.byref(t), d["_rowA"], _p(p1, _i32p), _p(p2, _i32p), ctypes.byref(rbb),
+                         None, None, None)
+    n = rbb.value
+    idx = np.zeros(max(n, 1), np.uint32)
+    cs = np.zeros(max(n, 1), np.int32)
+    ce = np.zeros(max(n, 1), np.int32)
+    L.oracle_prep_tables(ctypes.byref(t), d["_rowA"], _p(p1, _i32p), _p(p2, _i32p), ctypes.byref(rbb),
+                         idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _p(cs, _i32p), _p(ce, _i32p))
+    return dict(ptroffset1=p1[:T], ptroffset2=p2[:T], rowblkblock=n, rowblk_idx=idx[:n],
+                rowblk_colstart=cs[:n], rowblk_colstop=ce[:n])
+
+
+def row_lists(d):
+    L = lib()
+    t, keep = _to_struct(d)
+    T = int(d["tilenum"])
+    nzr = np.zeros(T + 1, np.int32)
+    tot = ctypes.c_int(0)
+    L.oracle_row_lists(ctypes.byref(t), d["_rowA"], _p(nzr, _i32p), None, None, ctypes.byref(tot))
+    n = tot.value
+    rid = np.zeros(max(n, 1), np.uint8)
+    rend = np.zeros(max(n, 1), np.uint8)
+    L.oracle_row_lists(ctypes.byref(t), d["_rowA"], _p(nzr, _i32p), _p(rid, _u8p), _p(rend, _u8p),
+                       ctypes.byref(tot))
+    return dict(nonzero_row_new=nzr, blockrowid_new=rid[:n], blockcsr_ptr_new=rend[:n])
+
+
+def encode(s):
+    L = lib()
+    s = np.ascontiguousarray(s, dtype=np.uint8)
+    n = s.size
+    res = np.zeros(n // 2 + 2, np.uint8)
+    L.oracle_encode(_p(s if n else np.zeros(1, np.uint8), _u8p), _p(res, _u8p), n, 0)
+    return res[: (n + 1) // 2]
+
+
+def decode(packed, n):
+    L = lib()
+    packed = np.ascontiguousarray(packed, dtype=np.uint8)
+    out = np.zeros(n + 1, np.uint8)
+    L.oracle_decode(_p(packed if packed.size else np.zeros(1, np.uint8), _u8p), _p(out, _u8p), n)
+    return out[:n]
+
+
+def tile_spmv(d, x):
+    L = lib()
+    t, keep = _to_struct(d)
+    xx = np.zeros(int(d["tilen"]) * 16 + 16)
+    xx[: len(x)] = x
+    y = np.zeros(int(d["tilem"]) * 16 + 16)
+    L.oracle_tile_spmv(ctypes.byref(t), d["_rowA"], _p(xx, _f64p), _p(y, _f64p))
+    return y[: d["_rowA"]]
+
+
+def csr_spmv(rowptr, colidx, val, x):
+    L = lib()
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    n = rowptr.size - 1
+    y = np.zeros(n)
+    L.oracle_csr_spmv(n, _p(rowptr, _i32p), _p(colidx, _i32p), _p(val, _f64p), _p(x, _f64p), _p(y, _f64p))
+    return y
+
+
+def _solve(name, rowptr, colidx, val, b, tol, maxit, want_hist=False):
+    L = lib()
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    n = rowptr.size - 1
+    x = np.zeros(n)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    hist = np.zeros(maxit + 2) if want_hist else None
+    st = getattr(L, name)(n, _p(rowptr, _i32p), _p(colidx, _i32p), _p(val, _f64p), _p(b, _f64p),
+                          _p(x, _f64p), float(tol), int(maxit), ctypes.byref(it), ctypes.byref(rel),
+                          _p(hist, _f64p) if want_hist else None)
+    out = dict(status=st, x=x, iterations=it.value, relres=rel.value)
+    if want_hist:
+        out["hist"] = hist[: it.value + 1]
+    return out
+
+
+def cg(rowptr, colidx, val, b, tol=1e-10, maxit=100000, want_hist=False):
+    """Textbook FP64 CG from x0 = 0 (Mille-feuille-cg.h:1188-1346)."""
+    return _solve("oracle_cg", rowptr, colidx, val, b, tol, maxit, want_hist)
+
+
+def bicgstab(rowptr, colidx, val, b, tol=1e-10, maxit=100000, want_hist=False):
+    """Textbook FP64 BiCGSTAB from x0 = 0 (Mille-feuille-bicg.h:562-822)."""
+    return _solve("oracle_bicgstab", rowptr, colidx, val, b, tol, maxit, want_hist)
+
+
+def dyn_code(delta):
+    return lib().oracle_dyn_code(float(delta))
+
+
+def dyn_class(p_old, p_new):
+    p_old = np.ascontiguousarray(p_old, np.float64)
+    p_new = np.ascontiguousarray(p_new, np.float64)
+    return lib().oracle_dyn_class(_p(p_old, _f64p), _p(p_new, _f64p), int(p_old.size))
+
+
+def static_class(vals):
+    vals = np.ascontiguousarray(vals, np.float64)
+    if vals.size == 0:
+        return 3
+    return lib().oracle_static_class(_p(vals, _f64p), int(vals.size))
+
+
+def static_class_np(vals):
+    """Per-value class (3 int8, 2 fp16, 1 fp32, 0 fp64) by round-trip exactness."""
+    v = np.asarray(vals, np.float64)
+    with np.errstate(all="ignore"):
+        is_i8 = (v == np.floor(v)) & (v >= -128) & (v <= 127)
+        h = v.astype(np.float16).astype(np.float64)
+        is_h = (h == v) & np.isfinite(v)
+        f = v.astype(np.float32).astype(np.float64)
+        is_f = f == v
+    c = np.zeros(v.shape, np.int8)
+    c[is_f] = 1
+    c[is_h] = 2
+    c[is_i8] = 3
+    return c
+
+
+def tile_classes(d):
+    """Static class of every tile (minimum over its values)."""
+    T = int(d["tilenum"])
+    per = static_class_np(d["Blockcsr_Val"])
+    off = d["tile_nnz"]
+    out = np.full(T, 3, np.int8)
+    for t in range(T):
+        a, b = off[t], off[t + 1]
+        if b > a:
+            out[t] = per[a:b].min()
+    return out
+
+
+def mixed_tile_spmv(d, x, classes, policy="mixed"):
+    """y = A x over the tile format with per-tile precision semantics.
+
+    policy "fp64": every tile in binary64.  policy "mixed": tiles of class 0 in
+    binary64; tiles of class 1/2/3 read their value at that precision, multiply by
+    x converted to binary32, product rounded to binary32, summed in binary64.
+    Row sums are accumulated over the tiles of a tile row in tile order.
+    """
+    rowA = d["_rowA"]
+    tp, tc, off = d["tile_ptr"], d["tile_columnidx"], d["tile_nnz"]
+    vals = d["Blockcsr_Val"]
+    cols = d["Tile_csr_Col"].astype(np.int64)
+    rowof = entry_rows(d)
+    xx = np.zeros(int(d["tilen"]) * 16 + 16)
+    xx[: len(x)] = x
+    x32 = xx.astype(np.float32)
+    y = np.zeros(int(d["tilem"]) * 16)
+    for bi in range(int(d["tilem"])):
+        for t in range(tp[bi], tp[bi + 1]):
+            a, b = off[t], off[t + 1]
+            if b == a:
+                continue
+            c = 0 if policy == "fp64" else int(classes[t])
+            gc = tc[t] * 16 + cols[a:b]
+            if c == 0:
+                prod = vals[a:b] * xx[gc]
+            else:
+                if c == 1:
+                    v = vals[a:b].astype(np.float32)
+                elif c == 2:
+                    v = vals[a:b].astype(np.float16).astype(np.float32)
+                else:
+                    v = vals[a:b].astype(np.float32)
+                prod = (v * x32[gc]).astype(np.float64)
+            np.add.at(y, bi * 16 + rowof[a:b], prod)
+    return y[:rowA]
+
+
+def entry_rows(d):
+    """Local row (0..15) of every stored entry, from the per-row start bytes."""
+    rowA = d["_rowA"]
+    out = np.zeros(int(d["csrsize"]), np.int64)
+    tp, off, cpo = d["tile_ptr"], d["tile_nnz"], d["csrptr_offset"]
+    ptr = d["Blockcsr_Ptr"].astype(np.int64)
+    tilem = int(d["tilem"])
+    for bi in range(tilem):
+        rl = rowA - (tilem - 1) * 16 if bi == tilem - 1 else 16
+        for t in range(tp[bi], tp[bi + 1]):
+            a, b = off[t], off[t + 1]
+            if rl == 1:
+                out[a:b] = 0
+                continue
+            starts = ptr[cpo[t]: cpo[t] + rl]
+            for ri in range(rl):
+                s = starts[ri]
+                e = starts[ri + 1] if ri < rl - 1 else b - a
+                out[a + s: a + e] = ri
+    return out
+
+
+def true_relres(rowptr, colidx, val, x, b):
+    """||b - A x|| / ||b|| in FP64 on the host (main-cg.cu:1010-1043)."""
+    r = np.asarray(b, np.float64) - csr_spmv(rowptr, colidx, val, x)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(r) / nb) if nb > 0 else float(np.linalg.norm(r))
+
+
+def ref_harness_path():
+    return HERE / "_ref" / "ref_harness"
+
+
+def build_ref(ref_src="/root/reference/src"):
+    """Compile the reference-header harness into oracle/_ref when the tree exists."""
+    if not os.path.isdir(ref_src):
+        return None
+    subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF_SRC={ref_src}"], check=True)
+    return ref_harness_path()
+
+
+def read_records(path):
+    """Parse the harness's record file into a dict of NumPy arrays."""
+    dts = {1: np.int32, 2: np.uint8, 3: np.int8, 4: np.float64, 5: np.float32, 6: np.uint32}
+    out = {}
+    raw = pathlib.Path(path).read_bytes()
+    pos = 0
+    while pos < len(raw):
+        ln = int(np.frombuffer(raw, np.int32, 1, pos)[0]); pos += 4
+        name = raw[pos: pos + ln].decode(); pos += ln
+        dt = int(np.frombuffer(raw, np.int32, 1, pos)[0]); pos += 4
+        cnt = int(np.frombuffer(raw, np.int64, 1, pos)[0]); pos += 8
+        t = np.dtype(dts[dt])
+        out[name] = np.frombuffer(raw, t, cnt, pos).copy()
+        pos += cnt * t.itemsize
+    return out
