@@ -1,0 +1,204 @@
+"""Benchmark: single-kernel CG to a 1e-10 relative residual on the 3D 7-point Poisson
+problem on a 128^3 grid (n = 2,097,152, nnz = 14,581,760), tile-mixed vs all-FP64,
+plus BiCGSTAB on the 128^3 convection-diffusion problem.
+
+One "step" is one complete solve of the tile-mixed CG (inputs resident in HBM, the
+matrix already in the tiled format).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-bicg]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "solve time to 1e-10 rel. residual + HBM GB/s, tile-mixed vs FP64 CG"
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bicg", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              f"described in DESIGN.md and not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from __graft_entry__ import load_package
+    mf = load_package()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    nx = args.grid
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    log(f"[bench] device {mf.device_name(0)}; building 3D Poisson {nx}^3")
+    rp, ci, va = mf.workloads.poisson3d(nx)
+    b_np, xs = mf.workloads.rhs_from_solution(rp, ci, va, seed=1)
+    n, nnz = rp.size - 1, ci.size
+    t0 = time.time()
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    t_conv = time.time() - t0
+    st = A.stats()
+    b = torch.as_tensor(b_np, device=dev)
+    log(f"[bench] n={n} nnz={nnz} tiles={st['tiles']} chunks={st['chunks']} "
+        f"classes={st['tiles_by_class']} conversion+upload {t_conv:.2f}s")
+
+    def run(prec, steps, warmup, solver="cg", mat=None, rhs=None):
+        mat = mat or A
+        rhs = rhs if rhs is not None else b
+        fn = mat.cg if solver == "cg" else mat.bicgstab
+        for _ in range(warmup):
+            fn(rhs, tol=1e-10, precision=prec)
+        torch.cuda.synchronize()
+        infos = []
+        tw = time.perf_counter()
+        for _ in range(steps):
+            x, info = fn(rhs, tol=1e-10, precision=prec)
+            infos.append(info)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - tw) / steps * 1e3
+        return x, infos, wall
+
+    x_m, inf_m, wall_m = run("mixed", args.steps, args.warmup)
+    log(f"[bench] mixed CG: iters={inf_m[-1]['iterations']} repl={inf_m[-1]['replacements']} "
+        f"kernel {np.median([i['time_ms'] for i in inf_m]):.3f} ms wall {wall_m:.3f} ms "
+        f"relres_true={inf_m[-1]['relres_true']:.3e}")
+    x_d, inf_d, wall_d = run("fp64", args.steps, args.warmup)
+    log(f"[bench] fp64  CG: iters={inf_d[-1]['iterations']} "
+        f"kernel {np.median([i['time_ms'] for i in inf_d]):.3f} ms wall {wall_d:.3f} ms "
+        f"relres_true={inf_d[-1]['relres_true']:.3e}")
+
+    # host FP64 check of the final residual (||b - A x|| / ||b||)
+    def host_relres(x):
+        xh = x.cpu().numpy()
+        r = b_np - mf.workloads.spmv(rp, ci, va, xh)
+        return float(np.linalg.norm(r) / np.linalg.norm(b_np))
+
+    rel_m, rel_d = host_relres(x_m), host_relres(x_d)
+    t_mixed = float(np.median([i["time_ms"] for i in inf_m]))
+    t_fp64 = float(np.median([i["time_ms"] for i in inf_d]))
+    it_m, it_d = inf_m[-1]["iterations"], inf_d[-1]["iterations"]
+    repl = inf_m[-1]["replacements"]
+    # algorithmic bytes of one launch: (M + 16 n) per iteration; each residual
+    # replacement and the final true-residual check re-stream the binary64 matrix
+    bytes_mixed = (st["alg_bytes_mixed"] + 16.0 * n) * it_m + (st["alg_bytes_fp64"] + 16.0 * n) * (repl + 1)
+    bytes_fp64 = (st["alg_bytes_fp64"] + 16.0 * n) * (it_d + 1)
+    gbs_m = bytes_mixed / (t_mixed * 1e-3) / 1e9
+    gbs_d = bytes_fp64 / (t_fp64 * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_bicg:
+        rp2, ci2, va2 = mf.workloads.convdiff3d(nx)
+        b2_np, _ = mf.workloads.rhs_from_solution(rp2, ci2, va2, seed=1)
+        A2 = mf.TiledMatrix(rp2, ci2, va2, class_policy="lossless")
+        st2 = A2.stats()
+        b2 = torch.as_tensor(b2_np, device=dev)
+        xb_m, ib_m, _ = run("mixed", max(1, args.steps // 2), 1, "bicg", A2, b2)
+        xb_d, ib_d, _ = run("fp64", max(1, args.steps // 2), 1, "bicg", A2, b2)
+        tb_m = float(np.median([i["time_ms"] for i in ib_m]))
+        tb_d = float(np.median([i["time_ms"] for i in ib_d]))
+        rb = b2_np - mf.workloads.spmv(rp2, ci2, va2, xb_m.cpu().numpy())
+        extra["bicgstab_convdiff"] = {
+            "mixed_ms": tb_m, "fp64_ms": tb_d, "speedup": tb_d / tb_m,
+            "iters_mixed": ib_m[-1]["iterations"], "iters_fp64": ib_d[-1]["iterations"],
+            "relres_true_mixed": float(np.linalg.norm(rb) / np.linalg.norm(b2_np)),
+            "tiles_by_class": st2["tiles_by_class"],
+            "alg_GBs_mixed": (2 * st2["alg_bytes_mixed"] + 32.0 * n) * ib_m[-1]["iterations"] / (tb_m * 1e-3) / 1e9,
+        }
+        log(f"[bench] BiCGSTAB convdiff: mixed {tb_m:.3f} ms ({ib_m[-1]['iterations']} it) "
+            f"fp64 {tb_d:.3f} ms ({ib_d[-1]['iterations']} it)")
+        A2.close()
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle  # the checker, timed as the CPU baseline
+        oracle.build()
+        # bounded sample: K iterations of the oracle's FP64 CG on the same system,
+        # scaled to the FP64 iteration count of the GPU solve
+        k = 2
+        while True:
+            tc = time.perf_counter()
+            res = oracle.cg(rp, ci, va, b_np, tol=0.0, maxit=k)
+            dt = time.perf_counter() - tc
+            if dt > args.cpu_seconds / 4 or k >= it_d:
+                break
+            k = min(it_d, int(k * max(2.0, args.cpu_seconds / 2 / max(dt, 1e-3))))
+        per_it = dt / k
+        cpu = {"value": per_it * it_d * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"oracle_cg (textbook FP64 CG, C, 1 thread) on the same 128^3 system: {k} "
+                         f"iterations timed ({dt:.1f} s), scaled to the {it_d} iterations of the FP64 solve",
+               "cpu": cpu_model(), "nproc": os.cpu_count()}
+        log(f"[bench] cpu baseline: {per_it * 1e3:.2f} ms/iteration -> {cpu['value']:.0f} ms per solve")
+
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_cg_poisson128_mixed.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(t_mixed, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_m, 4),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 vectors/scalars; tiles i8/f16/f32/f64 with f32 products",
+        "data": "synthetic (3D 7-point Poisson, b = A x*, x* ~ U[-1,1] seed 1)",
+        "config": {"workload": f"3D 7-point Poisson {nx}^3 single-kernel CG to 1e-10, tile-mixed vs FP64",
+                   "n": int(n), "nnz": int(nnz), "tiles": int(st["tiles"]), "tol": 1e-10,
+                   "tiles_by_class": st["tiles_by_class"]},
+        "fp64_ms": round(t_fp64, 4),
+        "speedup_vs_fp64": round(t_fp64 / t_mixed, 4),
+        "iterations": {"mixed": it_m, "fp64": it_d, "replacements": repl},
+        "relres_true_host": {"mixed": rel_m, "fp64": rel_d},
+        "alg_GBs": {"mixed": round(gbs_m, 1), "fp64": round(gbs_d, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs_m, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs_m / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "grid": inf_m[-1]["grid"],
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    A.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,310 @@
+/*
+ * mf.h -- C-ABI of the MI355X Mille-feuille solver library (libmf.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Device pointers are
+ * HIP device addresses; `stream` arguments are hipStream_t passed as void*
+ * (NULL = the default stream).  One mf_matrix per host thread at a time; no
+ * global mutable state is shared between handles.
+ *
+ * Reference interfaces each group replaces are cited inline
+ * (file:line in zhaijiaqi/Mille-feuille).
+ */
+#ifndef MF_H
+#define MF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_TILE 16
+
+typedef enum mf_status {
+    MF_SUCCESS = 0,
+    MF_ERR_INVALID = 1,     /* bad argument or malformed matrix */
+    MF_ERR_HIP = 2,         /* HIP runtime error */
+    MF_ERR_OCCUPANCY = 3,   /* persistent grid would not be co-resident (never oversubscribed) */
+    MF_NOT_CONVERGED = 4,   /* max_iter reached before the tolerance */
+    MF_BREAKDOWN = 5,       /* zero or non-finite p'Ap, rho, omega or t't */
+    MF_TIMEOUT = 6,         /* grid-barrier watchdog fired */
+    MF_ERR_ALLOC = 7,       /* host or device allocation failed */
+    MF_ERR_NO_DEVICE = 8,   /* no HIP device */
+    MF_ERR_TOO_LARGE = 9,   /* exceeds the single-kernel on-chip capacity */
+    MF_ERR_UNSUPPORTED = 10
+} mf_status;
+
+const char *mf_status_string(int status);
+
+/* ------------------------------------------------------------------------ */
+/* Tiled sparse format (host).                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Field-for-field mirror of the reference Tile_matrix (format.h:14-86) so a
+ * caller written against it keeps working.  mf_tile_create fills exactly the
+ * arrays the reference Tile_create fills; the COO/ELL/HYB/dense fields, which
+ * the reference never allocates, are left NULL / 0. */
+typedef struct mf_tile_matrix {
+    int tilem;
+    int tilen;
+    int tilenum;
+    int *tile_ptr;
+    int *tile_columnidx;
+    int *tile_nnz;
+    char *Format;
+    int *blknnz;
+    unsigned char *blknnznnz;
+    int *dnsrowptr;
+    int *dnscolptr;
+    char *tilewidth;
+    int *csr_offset;
+    int *csrptr_offset;
+    int *coo_offset;
+    int *ell_offset;
+    int *hyb_offset;
+    int *hyb_coocount;
+    int *dns_offset;
+    int *dnsrow_offset;
+    int *dnscol_offset;
+    int *new_coocount;
+    double *Blockcsr_Val;
+    float *Blockcsr_Val_Low;
+    unsigned char *Tile_csr_Col;
+    void *Blockcsr_Val_mix;
+    unsigned char *Blockcsr_Ptr;
+    unsigned char *csr_compressedIdx;
+    int csrsize;
+    int csrptrlen;
+    double *Blockcoo_Val;
+    float *Blockcoo_Val_Low;
+    unsigned char *coo_compressed_Idx;
+    int coosize;
+    double *Blockell_Val;
+    float *Blockell_Val_Low;
+    unsigned char *ell_compressedIdx;
+    int ellsize;
+    double *Blockhyb_Val;
+    float *Blockhyb_Val_Low;
+    unsigned char *hybIdx;
+    int hybsize;
+    int hybellsize;
+    int hybcoosize;
+    double *Blockdense_Val;
+    float *Blockdense_Val_Low;
+    int dnssize;
+    double *Blockdenserow_Val;
+    float *Blockdenserow_Val_Low;
+    char *denserowid;
+    int dnsrowsize;
+    double *Blockdensecol_Val;
+    float *Blockdensecol_Val_Low;
+    char *densecolid;
+    int dnscolsize;
+    int coototal;
+    double *deferredcoo_val;
+    float *deferredcoo_val_Low;
+    int *deferredcoo_colidx;
+    int *deferredcoo_ptr;
+    int rowblkblock;
+    unsigned int *flag_bal_tile_rowidx;
+    int *tile_bal_rowidx_colstart;
+    int *tile_bal_rowidx_colstop;
+    int *flag_tilerow_start;
+    int *flag_tilerow_stop;
+    int *tile_bal_rowidx_colstart_v2;
+    int *tile_bal_rowidx_colstop_v2;
+    int *map;
+} mf_tile_matrix;
+
+/* Replaces Tile_create (csr2block.h:267-422): CSR (rowA rows, colA columns) ->
+ * 16x16 CSR tiles, arrays bit-identical to the reference's.  val_low may be NULL
+ * (then (float)val).  Arrays are malloc'ed; release with mf_tile_destroy. */
+mf_status mf_tile_create(mf_tile_matrix *m, int rowA, int colA, int nnzA, const int *rowptr,
+                         const int *colidx, const double *val, const float *val_low);
+/* Frees exactly what mf_tile_create allocated (the reference's Tile_destroy,
+ * format.h:88-124, frees fields it never allocated). */
+void mf_tile_destroy(mf_tile_matrix *m);
+
+/* Replaces the table part of blockspmv_cpu (blockspmv_cpu.h:35-97).  Call with
+ * idx == NULL to query *rowblkblock; then pass arrays of that length. */
+mf_status mf_tile_prep_tables(const mf_tile_matrix *m, int rowA, int *ptroffset1, int *ptroffset2,
+                              int *rowblkblock, unsigned int *idx, int *colstart, int *colstop);
+
+/* Row-compressed per-tile lists of the reference drivers (main-cg.cu:182-237):
+ * nonzero_row_new[tilenum+1]; call with rowid == NULL to query *total. */
+mf_status mf_tile_row_lists(const mf_tile_matrix *m, int rowA, int *nonzero_row_new,
+                            unsigned char *rowid, unsigned char *rowend, int *total);
+
+/* 4-bit packing of local column indices (encode.h:29-72). */
+void mf_encode(const unsigned char *str, unsigned char *res, int len, int offset);
+void mf_decode(const unsigned char *str, unsigned char *res, int len);
+
+/* Value copies of the mixed drivers (main-cg-mixed.cu:2333-2350): f32 = RNE,
+ * f16 = RNE of the f32 value (binary16 bits), i8 = truncation toward zero with the
+ * range clamped to [-128,127] (the reference leaves out-of-range as undefined). */
+void mf_value_copies(const double *val, int64_t count, float *f32, uint16_t *f16, int8_t *i8);
+
+/* Storage-class policies of the tiled value stream. */
+enum {
+    MF_CLASS_LOSSLESS = 0,  /* lowest precision that holds every value of the tile exactly */
+    MF_CLASS_ROWSCALED = 1, /* lowest precision whose rounding error is <= 2^-24 * row max|a| */
+    MF_CLASS_ALL_FP64 = 2
+};
+/* Class codes: 0 = fp64, 1 = fp32, 2 = fp16, 3 = int8 (value representable exactly). */
+enum { MF_FP64 = 0, MF_FP32 = 1, MF_FP16 = 2, MF_INT8 = 3 };
+
+/* Static storage class of every tile of a CSR matrix, tiles in Tile_create order. */
+mf_status mf_tile_classes(int rowA, int colA, const int *rowptr, const int *colidx, const double *val,
+                          int policy, unsigned char *cls_out, int64_t cls_capacity, int64_t *ntiles);
+
+/* ------------------------------------------------------------------------ */
+/* Matrix I/O (biio semantics: biio.h:153-194, mmio_highlevel.h:151-321).    */
+/* ------------------------------------------------------------------------ */
+/* .mtx: symmetric/hermitian mirrored, pattern -> 1.0, columns kept in file order;
+ * .cbd: int rows, int cols, int64 nnz, int64 rowptr[rows+1], int col[nnz], f64 val[nnz].
+ * Arrays are malloc'ed; free with mf_free. */
+mf_status mf_read_matrix(const char *path, int *m, int *n, int *nnz, int **rowptr, int **colidx,
+                         double **val, int *is_symmetric);
+mf_status mf_write_cbd(const char *path, int m, int n, int nnz, const int *rowptr, const int *colidx,
+                       const double *val);
+void mf_free(void *p);
+
+/* ------------------------------------------------------------------------ */
+/* GPU-resident tiled matrix and solvers.                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct mf_matrix mf_matrix;
+
+enum { MF_PREC_FP64 = 0, MF_PREC_MIXED = 1 };
+
+typedef struct mf_matrix_options {
+    int class_policy;     /* MF_CLASS_* for the mixed value stream */
+    int device;           /* HIP device ordinal */
+    int keep_mixed;       /* build the low-precision value streams (1) or FP64 only (0) */
+} mf_matrix_options;
+
+typedef struct mf_matrix_stats {
+    int n_rows, n_cols;
+    int64_t nnz;
+    int64_t tiles;
+    int64_t chunks;
+    int tilem, tilen;
+    int64_t tiles_by_class[4];
+    int64_t nnz_by_class[4];
+    double alg_bytes_fp64;     /* M (f64) = sum over tiles of 12 + 9 * nnz */
+    double alg_bytes_mixed;    /* M (mixed) = sum over tiles of 12 + (s_v + 1) * nnz */
+    double device_bytes;       /* resident device footprint */
+} mf_matrix_stats;
+
+typedef struct mf_solve_options {
+    double tol;            /* target ||b - A x|| / ||b|| */
+    int max_iter;
+    int precision;         /* MF_PREC_FP64 or MF_PREC_MIXED */
+    int fixed_iterations;  /* reference mode: run exactly max_iter iterations */
+    int dynamic_precision; /* 0 off; 1 partial-convergence demotion (|dp| relative to ||p||_inf);
+                              2 the reference's absolute |dp| thresholds */
+    double replace_delta;  /* mixed: replace r by b - A x once ||r|| < delta * ||r|| at the last
+                              replacement (0 = default 1e-3) */
+    int max_replacements;  /* 0 = default 64 */
+    int grid;              /* 0 = autotuned; else workgroups (must be co-resident) */
+    int block;             /* 0 = default (1024) */
+    int variant;           /* 0 = default; 1 = textbook (3 grid syncs/iteration);
+                              2 = fused reductions (2 grid syncs/iteration) */
+    int use_x0;            /* x holds an initial guess (else x0 = 0) */
+    double watchdog_ms;    /* grid-barrier wait bound (0 = default 10000 ms) */
+    int transpose;         /* solve with A^T (same tile stream read transposed) */
+} mf_solve_options;
+
+typedef struct mf_solve_info {
+    int status;
+    int iterations;
+    int replacements;
+    int grid;
+    int block;
+    int rows_per_thread;
+    double relres;          /* recursive residual at exit */
+    double relres_true;     /* ||b - A x|| / ||b|| evaluated in FP64 on the GPU at exit */
+    double time_ms;         /* device time of the solve launch (hipEvent) */
+    double bytes_per_iter;  /* algorithmic bytes per iteration (M + 16 n; BiCGSTAB 2M + 32 n) */
+    double gbs;             /* bytes_per_iter * iterations / time */
+} mf_solve_info;
+
+void mf_solve_options_default(mf_solve_options *o);
+void mf_matrix_options_default(mf_matrix_options *o);
+
+/* Build tiles, classes and the device layout from a CSR matrix (host arrays) and
+ * upload it.  n_rows need not be a multiple of 16. */
+mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr, const int *colidx,
+                           const double *val, const mf_matrix_options *opt, void *stream,
+                           mf_matrix **out);
+mf_status mf_matrix_destroy(mf_matrix *A);
+mf_status mf_matrix_get_stats(const mf_matrix *A, mf_matrix_stats *st);
+/* Overwrite the values (same pattern, CSR order), e.g. for finite differences. */
+mf_status mf_matrix_update_values(mf_matrix *A, const double *val, void *stream);
+
+/* y = A x (or A^T x) on device pointers.  precision MF_PREC_MIXED applies the tile
+ * classes (binary32 products, FP64 sums); MF_PREC_FP64 reads the FP64 stream. */
+mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int precision, int transpose,
+                  void *stream);
+
+/* Single-kernel CG (replaces cg_solve_inc/sync/reduce, main-cg.cu:19,405,1110 and
+ * main-cg-mixed.cu:2248).  d_b, d_x: device vectors of n_rows doubles. */
+mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
+                      mf_solve_info *info, void *stream);
+/* Single-kernel BiCGSTAB (replaces bicgstab/_sync/_inc, main-bicg.cu:13,711,1336 and
+ * main-bicg-mixed.cu:1355). */
+mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
+                            mf_solve_info *info, void *stream);
+/* Host-vector conveniences (copy in, solve, copy out, synchronise). */
+mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,
+                           mf_solve_info *info);
+mf_status mf_bicgstab_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,
+                                 mf_solve_info *info);
+
+/* Multi-kernel fallback (one launch per BLAS-1 / SpMV step, host loop): the same
+ * recurrences without a co-residency requirement (main-cg.cu:258-292,
+ * main-bicg.cu:1619-1656). */
+mf_status mf_cg_solve_multikernel(mf_matrix *A, const double *d_b, double *d_x,
+                                  const mf_solve_options *opt, mf_solve_info *info, void *stream);
+mf_status mf_bicgstab_solve_multikernel(mf_matrix *A, const double *d_b, double *d_x,
+                                        const mf_solve_options *opt, mf_solve_info *info, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched solves: many independent systems in one persistent launch.        */
+/* ------------------------------------------------------------------------ */
+typedef struct mf_batch mf_batch;
+
+/* count systems given as separate CSR arrays (host).  Vectors of system s live at
+ * offset row_offset[s] of one concatenated device vector (mf_batch_row_offsets). */
+mf_status mf_batch_create(int count, const int *n_rows, const int *const *rowptr,
+                          const int *const *colidx, const double *const *val,
+                          const mf_matrix_options *opt, void *stream, mf_batch **out);
+mf_status mf_batch_destroy(mf_batch *B);
+mf_status mf_batch_row_offsets(const mf_batch *B, int64_t *offsets /* count+1 */);
+/* One persistent launch, per-system barrier groups, early retirement.  When
+ * use_graph != 0 the launch is captured once into a hipGraph and replayed. */
+mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
+                              int use_graph, mf_solve_info *per_system, double *time_ms, void *stream);
+/* Workgroups per system chosen by the autotuner (written after a tuned solve). */
+mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline (OpenMP): tiled SpMV and host CG / BiCGSTAB loops, FP64.     */
+/* ------------------------------------------------------------------------ */
+mf_status mf_cpu_tile_spmv(int n_rows, int n_cols, const int *rowptr, const int *colidx,
+                           const double *val, const double *x, double *y, int threads);
+mf_status mf_cpu_cg(int n_rows, const int *rowptr, const int *colidx, const double *val,
+                    const double *b, double *x, double tol, int max_iter, int threads,
+                    int *iterations, double *relres);
+mf_status mf_cpu_bicgstab(int n_rows, const int *rowptr, const int *colidx, const double *val,
+                          const double *b, double *x, double tol, int max_iter, int threads,
+                          int *iterations, double *relres);
+
+/* Library / device information. */
+int mf_device_count(void);
+mf_status mf_device_name(int device, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MF_H */

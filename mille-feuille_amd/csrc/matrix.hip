@@ -1,0 +1,410 @@
+// GPU-resident tiled matrix: host-side construction of the device layout from the
+// tile structure, upload, partitioning for the persistent kernels, and the
+// standalone SpMV.
+//
+// Device layout (HBM), derived from the Tile_create order:
+//   chunks  int4[C]  16-entry pieces of tiles {e0, colbase, rowbase, (cnt-1)|class<<4}
+//   eidx    u8[nnz]  (local row << 4) | local column of every entry
+//   v64     f64[nnz] all values;  v32/v16/v8: the values of class 1/2/3 tiles at that
+//                    precision (only those entries are ever read)
+// A tile row's chunks are contiguous, so a workgroup owning a contiguous range of
+// tile rows owns a contiguous range of chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "mf_internal.h"
+#include "solver_internal.h"
+
+using namespace mf;
+
+namespace mf {
+
+const char *hip_err_where = nullptr;
+
+void free_dev(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s)
+{
+    *dst = nullptr;
+    if (bytes == 0) return MF_SUCCESS;
+    if (hipMalloc(dst, bytes) != hipSuccess) {
+        *dst = nullptr;
+        return MF_ERR_ALLOC;
+    }
+    if (src && hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return MF_ERR_HIP;
+    return MF_SUCCESS;
+}
+
+DeviceLayout::~DeviceLayout() { release(); }
+
+void DeviceLayout::release()
+{
+    free_dev(chunks);
+    free_dev(eidx);
+    free_dev(v64);
+    free_dev(v32);
+    free_dev(v16);
+    free_dev(v8);
+    for (auto &kv : parts) {
+        free_dev(kv.second.wg_tr);
+        free_dev(kv.second.wg_chunk);
+    }
+    parts.clear();
+    chunks = nullptr;
+    eidx = nullptr;
+    v64 = nullptr;
+    v32 = nullptr;
+    v16 = nullptr;
+    v8 = nullptr;
+}
+
+static mf_status build_layout(const TileStruct &ts, const double *val, const std::vector<uint8_t> &cls,
+                              bool keep_mixed, hipStream_t s, DeviceLayout &L)
+{
+    const int64_t T = ts.tiles();
+    if (ts.nnz >= INT32_MAX || (int64_t)ts.tilen * kTile >= INT32_MAX) return MF_ERR_TOO_LARGE;
+    L.n_rows = ts.rows;
+    L.n_cols = ts.cols;
+    L.tilem = ts.tilem;
+    L.nnz = ts.nnz;
+    L.tiles = T;
+    // chunks per tile row
+    L.tr_chunk.assign(ts.tilem + 1, 0);
+    for (int bi = 0; bi < ts.tilem; bi++) {
+        int64_t c = 0;
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+            c += (ts.tile_off[t + 1] - ts.tile_off[t] + 15) / 16;
+        L.tr_chunk[bi + 1] = L.tr_chunk[bi] + c;
+    }
+    const int64_t C = L.tr_chunk[ts.tilem];
+    L.nchunks = C;
+    std::vector<int4> ch(C > 0 ? C : 1);
+    std::vector<uint8_t> eid(ts.nnz > 0 ? ts.nnz : 1);
+    std::vector<double> v64(ts.nnz > 0 ? ts.nnz : 1);
+    std::vector<float> v32;
+    std::vector<uint16_t> v16;
+    std::vector<int8_t> v8;
+    bool any_lowp = false;
+    for (int c = 0; c < 4; c++) L.tiles_by_class[c] = L.nnz_by_class[c] = 0;
+    for (int64_t t = 0; t < T; t++) {
+        L.tiles_by_class[cls[t]]++;
+        L.nnz_by_class[cls[t]] += ts.tile_off[t + 1] - ts.tile_off[t];
+        if (cls[t] != MF_FP64) any_lowp = true;
+    }
+    if (keep_mixed && any_lowp) {
+        v32.assign(ts.nnz, 0.f);
+        v16.assign(ts.nnz, 0);
+        v8.assign(ts.nnz, 0);
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int bi = 0; bi < ts.tilem; bi++) {
+        int64_t c = L.tr_chunk[bi];
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++) {
+            const int64_t a = ts.tile_off[t], b = ts.tile_off[t + 1];
+            for (int64_t e0 = a; e0 < b; e0 += 16, c++) {
+                int cnt = (int)std::min<int64_t>(16, b - e0);
+                ch[c] = make_int4((int)e0, ts.tile_col[t] * kTile, bi * kTile, (cnt - 1) | (cls[t] << 4));
+            }
+            for (int64_t e = a; e < b; e++) {
+                eid[e] = (uint8_t)((ts.lrow[e] << 4) | ts.lcol[e]);
+                double v = val[ts.src[e]];
+                v64[e] = v;
+                if (!v32.empty()) {
+                    switch (cls[t]) {
+                    case MF_FP32: v32[e] = (float)v; break;
+                    case MF_FP16: v16[e] = f64_to_f16_rne(v); break;
+                    case MF_INT8: v8[e] = (int8_t)v; break;
+                    default: break;
+                    }
+                }
+            }
+        }
+    }
+    L.has_lowp = !v32.empty();
+    L.need_p64_mixed = L.tiles_by_class[MF_FP64] > 0;
+    mf_status st;
+    if ((st = upload((void **)&L.chunks, ch.data(), sizeof(int4) * C, s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&L.eidx, eid.data(), ts.nnz, s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&L.v64, v64.data(), sizeof(double) * ts.nnz, s)) != MF_SUCCESS) return st;
+    if (L.has_lowp) {
+        if ((st = upload((void **)&L.v32, v32.data(), sizeof(float) * ts.nnz, s)) != MF_SUCCESS) return st;
+        if ((st = upload((void **)&L.v16, v16.data(), sizeof(uint16_t) * ts.nnz, s)) != MF_SUCCESS) return st;
+        if ((st = upload((void **)&L.v8, v8.data(), ts.nnz, s)) != MF_SUCCESS) return st;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
+    // algorithmic bytes of one pass over the value stream (12 B per tile + value + 1 B column)
+    static const double sv[4] = {8, 4, 2, 1};
+    L.alg_bytes_f64 = 12.0 * (double)T + 9.0 * (double)ts.nnz;
+    L.alg_bytes_mixed = 12.0 * (double)T;
+    for (int c = 0; c < 4; c++) L.alg_bytes_mixed += (sv[c] + 1.0) * (double)L.nnz_by_class[c];
+    L.device_bytes = (double)C * 16 + (double)ts.nnz * (1 + 8 + (L.has_lowp ? 7 : 0));
+    return MF_SUCCESS;
+}
+
+// Contiguous tile-row ranges for G workgroups balancing chunks (+1 per tile row for
+// the vector work), cached per G on the device.
+mf_status DeviceLayout::partition(int G, hipStream_t s, const Partition **out)
+{
+    auto it = parts.find(G);
+    if (it != parts.end()) {
+        *out = &it->second;
+        return MF_SUCCESS;
+    }
+    Partition P;
+    P.G = G;
+    std::vector<int> tr(G + 1, 0);
+    std::vector<long long> cc(G + 1, 0);
+    const double total = (double)nchunks + (double)tilem;
+    if (G > tilem) return MF_ERR_INVALID;
+    int bi = 0;
+    for (int w = 1; w < G; w++) {
+        const double target = total * w / G;
+        while (bi < tilem && (double)tr_chunk[bi + 1] + (bi + 1) <= target) bi++;
+        // every workgroup keeps at least one tile row
+        bi = std::min(std::max(bi, tr[w - 1] + 1), tilem - (G - w));
+        tr[w] = bi;
+    }
+    tr[G] = tilem;
+    int rows_cap = 0;
+    for (int w = 0; w < G; w++) {
+        cc[w] = tr_chunk[tr[w]];
+        int r0 = tr[w] * kTile, r1 = std::min(tr[w + 1] * kTile, n_rows);
+        rows_cap = std::max(rows_cap, std::max(0, r1 - r0));
+    }
+    cc[G] = tr_chunk[tilem];
+    P.rows_cap = rows_cap;
+    mf_status st;
+    if ((st = upload((void **)&P.wg_tr, tr.data(), sizeof(int) * (G + 1), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&P.wg_chunk, cc.data(), sizeof(long long) * (G + 1), s)) != MF_SUCCESS) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
+    auto res = parts.emplace(G, P);
+    *out = &res.first->second;
+    return MF_SUCCESS;
+}
+
+mfk::DevMatrix DeviceLayout::dev(const Partition &P) const
+{
+    mfk::DevMatrix d;
+    d.chunks = chunks;
+    d.eidx = eidx;
+    d.v64 = v64;
+    d.v32 = v32;
+    d.v16 = (const _Float16 *)v16;
+    d.v8 = v8;
+    d.wg_tr = P.wg_tr;
+    d.wg_chunk = P.wg_chunk;
+    d.n = n_rows;
+    d.rows_cap = P.rows_cap;
+    return d;
+}
+
+mf_status Workspace::ensure(int64_t n, int G, hipStream_t s)
+{
+    (void)s;
+    if (n > cap_n) {
+        free_dev(p64);
+        free_dev(p32);
+        free_dev(v64);
+        free_dev(v32);
+        free_dev(s64);
+        free_dev(s32);
+        free_dev(rhat);
+        p64 = v64 = s64 = rhat = nullptr;
+        p32 = v32 = s32 = nullptr;
+        size_t nn = (size_t)std::max<int64_t>(n, 1) + 64;
+        if (hipMalloc((void **)&p64, nn * 8) != hipSuccess || hipMalloc((void **)&p32, nn * 4) != hipSuccess ||
+            hipMalloc((void **)&v64, nn * 8) != hipSuccess || hipMalloc((void **)&v32, nn * 4) != hipSuccess ||
+            hipMalloc((void **)&s64, nn * 8) != hipSuccess || hipMalloc((void **)&s32, nn * 4) != hipSuccess ||
+            hipMalloc((void **)&rhat, nn * 8) != hipSuccess)
+            return MF_ERR_ALLOC;
+        cap_n = n;
+    }
+    if (G > cap_g) {
+        free_dev(recs);
+        if (hipMalloc((void **)&recs, sizeof(unsigned long long) * 2 * 2 * mfk::kMaxRed * (size_t)G) != hipSuccess)
+            return MF_ERR_ALLOC;
+        cap_g = G;
+    }
+    if (!ctl) {
+        if (hipMalloc((void **)&ctl, 256) != hipSuccess) return MF_ERR_ALLOC;
+        if (hipMalloc((void **)&info, sizeof(double) * 32) != hipSuccess) return MF_ERR_ALLOC;
+    }
+    return MF_SUCCESS;
+}
+
+Workspace::~Workspace()
+{
+    free_dev(p64);
+    free_dev(p32);
+    free_dev(v64);
+    free_dev(v32);
+    free_dev(s64);
+    free_dev(s32);
+    free_dev(rhat);
+    free_dev(recs);
+    free_dev(ctl);
+    free_dev(info);
+}
+
+}  // namespace mf
+
+// ---------------------------------------------------------------------------
+// standalone SpMV: one workgroup per partition slot, LDS accumulation
+// ---------------------------------------------------------------------------
+namespace mfk {
+
+template <bool MIXED>
+__global__ __launch_bounds__(1024) void spmv_kernel(DevMatrix A, const double *__restrict__ x64,
+                                                    const float *__restrict__ x32, double *__restrict__ y)
+{
+    extern __shared__ double q[];
+    const int w = blockIdx.x;
+    const int tr0 = A.wg_tr[w], tr1 = A.wg_tr[w + 1];
+    const int row0 = tr0 * kTile, row1 = min(tr1 * kTile, A.n);
+    for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) q[i] = 0.0;
+    __syncthreads();
+    spmv_chunks<MIXED>(A, A.wg_chunk[w], A.wg_chunk[w + 1], row0, x64, x32, q);
+    __syncthreads();
+    for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
+}
+
+__global__ void to_f32_kernel(const double *__restrict__ x, float *__restrict__ y, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+
+}  // namespace mfk
+
+extern "C" void mf_matrix_options_default(mf_matrix_options *o)
+{
+    o->class_policy = MF_CLASS_LOSSLESS;
+    o->device = 0;
+    o->keep_mixed = 1;
+}
+
+extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr, const int *colidx,
+                                      const double *val, const mf_matrix_options *opt, void *stream,
+                                      mf_matrix **out)
+{
+    if (!out) return MF_ERR_INVALID;
+    *out = nullptr;
+    mf_matrix_options o;
+    mf_matrix_options_default(&o);
+    if (opt) o = *opt;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MF_ERR_NO_DEVICE;
+    if (o.device < 0 || o.device >= ndev) return MF_ERR_INVALID;
+    if (hipSetDevice(o.device) != hipSuccess) return MF_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    mf_matrix *A = new mf_matrix();
+    A->device = o.device;
+    A->class_policy = o.class_policy;
+    mf_status st = build_tiles(n_rows, n_cols, rowptr, colidx, A->ts);
+    if (st == MF_SUCCESS) {
+        tile_classes(A->ts, rowptr, val, o.class_policy, A->cls);
+        st = build_layout(A->ts, val, A->cls, o.keep_mixed != 0, s, A->L);
+    }
+    if (st == MF_SUCCESS) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, o.device) != hipSuccess) st = MF_ERR_HIP;
+        A->num_cus = prop.multiProcessorCount;
+        A->max_lds = (int)prop.maxSharedMemoryPerMultiProcessor;
+        if (A->max_lds <= 0) A->max_lds = 65536;
+    }
+    if (st != MF_SUCCESS) {
+        delete A;
+        return st;
+    }
+    *out = A;
+    return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_matrix_destroy(mf_matrix *A)
+{
+    if (!A) return MF_ERR_INVALID;
+    (void)hipSetDevice(A->device);
+    delete A;
+    return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_matrix_get_stats(const mf_matrix *A, mf_matrix_stats *st)
+{
+    if (!A || !st) return MF_ERR_INVALID;
+    memset(st, 0, sizeof(*st));
+    st->n_rows = A->L.n_rows;
+    st->n_cols = A->L.n_cols;
+    st->nnz = A->L.nnz;
+    st->tiles = A->L.tiles;
+    st->chunks = A->L.nchunks;
+    st->tilem = A->ts.tilem;
+    st->tilen = A->ts.tilen;
+    for (int c = 0; c < 4; c++) {
+        st->tiles_by_class[c] = A->L.tiles_by_class[c];
+        st->nnz_by_class[c] = A->L.nnz_by_class[c];
+    }
+    st->alg_bytes_fp64 = A->L.alg_bytes_f64;
+    st->alg_bytes_mixed = A->L.alg_bytes_mixed;
+    st->device_bytes = A->L.device_bytes;
+    return MF_SUCCESS;
+}
+
+namespace mf {
+int spmv_grid(const mf_matrix *A)
+{
+    int G = std::max(1, std::min<int>(A->num_cus, A->L.tilem));
+    return G;
+}
+}  // namespace mf
+
+extern "C" mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int precision, int transpose,
+                             void *stream)
+{
+    if (!A || !d_x || !d_y) return MF_ERR_INVALID;
+    if (transpose) return MF_ERR_UNSUPPORTED;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const bool mixed = precision == MF_PREC_MIXED && A->L.has_lowp;
+    int G = spmv_grid(A);
+    const Partition *P;
+    mf_status st = A->L.partition(G, s, &P);
+    if (st != MF_SUCCESS) return st;
+    if ((size_t)P->rows_cap * 8 > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
+    if ((st = A->ws.ensure(A->L.n_cols, G, s)) != MF_SUCCESS) return st;
+    mfk::DevMatrix dm = A->L.dev(*P);
+    size_t lds = (size_t)std::max(P->rows_cap, 1) * 8;
+    if (mixed) {
+        hipLaunchKernelGGL(mfk::to_f32_kernel, dim3(256), dim3(256), 0, s, d_x, A->ws.p32, A->L.n_cols);
+        (void)hipFuncSetAttribute((const void *)mfk::spmv_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(mfk::spmv_kernel<true>, dim3(G), dim3(1024), lds, s, dm, d_x, A->ws.p32, d_y);
+    } else {
+        (void)hipFuncSetAttribute((const void *)mfk::spmv_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(mfk::spmv_kernel<false>, dim3(G), dim3(1024), lds, s, dm, d_x, A->ws.p32, d_y);
+    }
+    return hipGetLastError() == hipSuccess ? MF_SUCCESS : MF_ERR_HIP;
+}
+
+extern "C" int mf_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" mf_status mf_device_name(int device, char *buf, int len)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MF_ERR_HIP;
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return MF_SUCCESS;
+}

@@ -1,0 +1,394 @@
+// Single-kernel (persistent) BiCGSTAB on the tiled matrix.
+//
+// Recurrence of Mille-feuille-bicg.h:562-822 (van der Vorst), host init of
+// main-bicg.cu:534-549 (x0 = 0 or the caller's x0, r = b - A x0, rhat = r, p = r,
+// rho = rhat'r):
+//   v = A p; alpha = rho / rhat'v; s = r - alpha v; t = A s; omega = t's / t't;
+//   x += alpha p + omega s; r = s - omega t; rho' = rhat'r;
+//   beta = (rho' / rho)(alpha / omega); p = r + beta (p - omega v).
+// Four grid syncs per iteration: rhat'v; publish s; {t's, t't, rhat's, rhat't}
+// (rho' = rhat's - omega rhat't); publish p with the directly summed r'r for the
+// convergence test.  Layout as in the CG kernel: r, p, v, s in registers, x and the
+// SpMV result in LDS, rhat in HBM (read-only), p and s published for the gathers.
+// Mixed precision: both SpMVs read the tiles at their storage class with binary32
+// products; residual replacement r = b - A x (binary64 stream) with rho recomputed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_common.h"
+#include "launch_common.h"
+#include "mf_internal.h"
+#include "solver_internal.h"
+
+namespace mfk {
+
+struct BicgArgs {
+    DevMatrix A;
+    GridSync gs;
+    const double *b;
+    double *x;
+    double *rhat;
+    double *p64;
+    float *p32;
+    double *s64;
+    float *s32;
+    double *info;  // [0] iterations [1] r'r [2] b'b [3] true r'r [4] status [5] replacements
+    double tol2, delta2;
+    int maxit, fixed, use_x0, need64, need32, max_repl;
+};
+
+enum { BST_OK = 0, BST_NOTCONV = 4, BST_BREAK = 5, BST_TIMEOUT = 6 };
+
+template <int RPT, int BLK>
+__device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, const double (&v)[RPT], double *d64,
+                                        float *d32)
+{
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        if (a.need64) d64[row] = v[k];
+        if (a.need32) d32[row] = (float)v[k];
+    }
+}
+
+// r <- b - A x (binary64 stream); returns grid {r'r, rhat'r}.
+template <int RPT, int BLK>
+__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, long long c0, long long c1, int wg,
+                                   double *q, double *xs, double *scratch, unsigned &epoch, double (&r)[RPT],
+                                   double &rr, double &rho)
+{
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+    double dummy[1] = {0.0};
+    if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
+    zero_q<RPT, BLK>(q, R);
+    __syncthreads();
+    spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, c0, c1, R.row0, a.x, nullptr, q);
+    __syncthreads();
+    double part[2] = {0.0, 0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        r[k] = a.b[row] - q[R.loc(k)];
+        part[0] += r[k] * r[k];
+        part[1] += a.rhat[row] * r[k];
+    }
+    if (!grid_sum<2>(a.gs, part, epoch, false, wg, scratch)) return false;
+    rr = part[0];
+    rho = part[1];
+    return true;
+}
+
+template <int RPT, int MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
+{
+    extern __shared__ double smem[];
+    double *scratch = smem;
+    double *q = smem + 64;
+    double *xs = smem + 64 + a.A.rows_cap;
+    const int G = a.gs.G;
+    const int wg = logical_wg(blockIdx.x, G);
+    const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
+    Rows<RPT, BLK> R;
+    R.row0 = tr0 * kTile;
+    R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
+    R.refresh();
+    const long long c0 = a.A.wg_chunk[wg], c1 = a.A.wg_chunk[wg + 1];
+    constexpr int U = RPT >= 8 ? 2 : 4;
+    unsigned epoch = 0;
+
+    double r[RPT], p[RPT], v[RPT], sv[RPT];
+    if (a.use_x0) {
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<false, U>(a.A, c0, c1, R.row0, a.x, nullptr, q);
+        __syncthreads();
+    }
+    double part[2] = {0.0, 0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        r[k] = p[k] = v[k] = sv[k] = 0.0;
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        const double bk = a.b[row];
+        xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
+        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        p[k] = r[k];
+        a.rhat[row] = r[k];
+        part[0] += r[k] * r[k];
+        part[1] += bk * bk;
+    }
+    publish<RPT, BLK>(a, R, p, a.p64, a.p32);
+    int status = BST_NOTCONV, it = 0, repl = 0;
+    double rr = 0, bb = 0, rho = 0, rbase = 0;
+    if (!grid_sum<2>(a.gs, part, epoch, true, wg, scratch)) {
+        status = BST_TIMEOUT;
+        goto done;
+    }
+    rr = part[0];
+    bb = part[1];
+    rho = rr;  // rhat = r
+    rbase = rr;
+    if (bb == 0.0) {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) xs[R.loc(k)] = 0.0;
+        rr = 0.0;
+        status = BST_OK;
+        goto done;
+    }
+    for (;;) {
+        if (!a.fixed) {
+            bool conv = rr <= a.tol2 * bb;
+            if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
+                if (!bicg_true_residual<RPT, BLK>(a, R, c0, c1, wg, q, xs, scratch, epoch, r, rr, rho)) {
+                    status = BST_TIMEOUT;
+                    break;
+                }
+                repl++;
+                rbase = rr;
+                conv = rr <= a.tol2 * bb;
+            }
+            if (conv) {
+                status = BST_OK;
+                break;
+            }
+        }
+        if (it >= a.maxit) {
+            status = a.fixed ? BST_OK : BST_NOTCONV;
+            break;
+        }
+        // v = A p ; alpha = rho / rhat'v
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<MIXED, U>(a.A, c0, c1, R.row0, a.p64, a.p32, q);
+        __syncthreads();
+        double d1[1] = {0.0};
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!R.ok(k)) continue;
+            v[k] = q[R.loc(k)];
+            d1[0] += a.rhat[R.row0 + R.loc(k)] * v[k];
+        }
+        if (!grid_sum<1>(a.gs, d1, epoch, false, wg, scratch)) {
+            status = BST_TIMEOUT;
+            break;
+        }
+        if (!(d1[0] != 0.0) || !isfinite(d1[0])) {
+            status = BST_BREAK;
+            break;
+        }
+        const double alpha = rho / d1[0];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) sv[k] = r[k] - alpha * v[k];
+        publish<RPT, BLK>(a, R, sv, a.s64, a.s32);
+        double z[1] = {0.0};
+        if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
+            status = BST_TIMEOUT;
+            break;
+        }
+        // t = A s ; omega = t's / t't ; rho' = rhat's - omega rhat't
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<MIXED, U>(a.A, c0, c1, R.row0, a.s64, a.s32, q);
+        __syncthreads();
+        double d4[4] = {0.0, 0.0, 0.0, 0.0};
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!R.ok(k)) continue;
+            const double tk = q[R.loc(k)];
+            const double rh = a.rhat[R.row0 + R.loc(k)];
+            d4[0] += tk * sv[k];
+            d4[1] += tk * tk;
+            d4[2] += rh * sv[k];
+            d4[3] += rh * tk;
+        }
+        if (!grid_sum<4>(a.gs, d4, epoch, false, wg, scratch)) {
+            status = BST_TIMEOUT;
+            break;
+        }
+        if (!(d4[1] != 0.0) || !isfinite(d4[1])) {
+            status = BST_BREAK;
+            break;
+        }
+        const double omega = d4[0] / d4[1];
+        const double rho_new = d4[2] - omega * d4[3];
+        if (!(omega != 0.0) || !(rho != 0.0) || !isfinite(rho_new)) {
+            status = BST_BREAK;
+            break;
+        }
+        const double beta = (rho_new / rho) * (alpha / omega);
+        double e[1] = {0.0};
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!R.ok(k)) continue;
+            const double tk = q[R.loc(k)];
+            xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
+            r[k] = sv[k] - omega * tk;
+            p[k] = r[k] + beta * (p[k] - omega * v[k]);
+            e[0] += r[k] * r[k];
+        }
+        publish<RPT, BLK>(a, R, p, a.p64, a.p32);
+        if (!grid_sum<1>(a.gs, e, epoch, true, wg, scratch)) {
+            status = BST_TIMEOUT;
+            break;
+        }
+        rr = e[0];
+        rho = rho_new;
+        it++;
+    }
+done:
+    double rt = -1.0;
+    if (status != BST_TIMEOUT) {
+        double rtr[RPT];
+        double rr_t = 0.0, rho_t = 0.0;
+        if (bicg_true_residual<RPT, BLK>(a, R, c0, c1, wg, q, xs, scratch, epoch, rtr, rr_t, rho_t)) rt = rr_t;
+        else status = BST_TIMEOUT;
+    } else {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+    }
+    if (wg == 0 && threadIdx.x == 0) {
+        a.info[0] = (double)it;
+        a.info[1] = rr;
+        a.info[2] = bb;
+        a.info[3] = rt;
+        a.info[4] = (double)status;
+        a.info[5] = (double)repl;
+    }
+}
+
+}  // namespace mfk
+
+namespace mf {
+
+using BicgKernel = void (*)(mfk::BicgArgs);
+
+template <int MIXED>
+static BicgKernel pick_bicg(int rpt)
+{
+    switch (rpt) {
+    case 1: return mfk::bicg_kernel<1, MIXED, 1024>;
+    case 2: return mfk::bicg_kernel<2, MIXED, 1024>;
+    case 4: return mfk::bicg_kernel<4, MIXED, 1024>;
+    case 8: return mfk::bicg_kernel<8, MIXED, 1024>;
+    default: return nullptr;
+    }
+}
+
+mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *o, mf_solve_info *info,
+                      hipStream_t s)
+{
+    const bool mixed = o->precision == MF_PREC_MIXED && A->L.has_lowp;
+    if (A->L.tilem == 0) return MF_ERR_INVALID;
+    const int G = choose_grid(A, o, (3LL << 8) | (mixed ? 1 : 0));
+    if (G > A->L.tilem) return MF_ERR_INVALID;
+    const Partition *P;
+    mf_status st = A->L.partition(G, s, &P);
+    if (st != MF_SUCCESS) return st;
+    int rpt = 0, B = 0;
+    if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
+    const size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
+    BicgKernel k = mixed ? pick_bicg<1>(rpt) : pick_bicg<0>(rpt);
+    if (!k) return MF_ERR_TOO_LARGE;
+    if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
+    mfk::BicgArgs a;
+    a.A = A->L.dev(*P);
+    a.gs.recs = A->ws.recs;
+    a.gs.ctl = A->ws.ctl;
+    a.gs.timeout = watchdog_cycles(o->watchdog_ms);
+    a.gs.G = G;
+    a.b = d_b;
+    a.x = d_x;
+    a.rhat = A->ws.rhat;
+    a.p64 = A->ws.p64;
+    a.p32 = A->ws.p32;
+    a.s64 = A->ws.s64;
+    a.s32 = A->ws.s32;
+    a.info = A->ws.info;
+    a.tol2 = o->tol * o->tol;
+    const double dl = o->replace_delta > 0 ? o->replace_delta : 1e-3;
+    a.delta2 = dl * dl;
+    a.maxit = o->max_iter;
+    a.fixed = o->fixed_iterations;
+    a.use_x0 = o->use_x0;
+    a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
+    a.need32 = mixed ? 1 : 0;
+    a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
+    float ms = 0;
+    double h[32];
+    if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;
+    const double bb = h[2];
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->iterations = (int)h[0];
+        info->status = (int)h[4];
+        info->replacements = (int)h[5];
+        info->relres = bb > 0 ? std::sqrt(std::max(h[1], 0.0) / bb) : 0.0;
+        info->relres_true = (h[3] >= 0 && bb > 0) ? std::sqrt(h[3] / bb) : (h[3] >= 0 ? std::sqrt(h[3]) : -1.0);
+        info->time_ms = ms;
+        info->grid = G;
+        info->block = B;
+        info->rows_per_thread = rpt;
+        const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
+        info->bytes_per_iter = 2.0 * M + 32.0 * A->L.n_rows;
+        info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;
+    }
+    return (mf_status)(int)h[4];
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
+                                       mf_solve_info *info, void *stream)
+{
+    if (!A || !d_b || !d_x) return MF_ERR_INVALID;
+    if (A->L.n_rows != A->L.n_cols) return MF_ERR_INVALID;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    mf_solve_options o;
+    mf_solve_options_default(&o);
+    if (opt) o = *opt;
+    if (o.transpose) return MF_ERR_UNSUPPORTED;
+    return bicg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
+}
+
+extern "C" mf_status mf_bicgstab_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,
+                                            mf_solve_info *info)
+{
+    if (!A || !b || !x) return MF_ERR_INVALID;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    const size_t n = (size_t)A->L.n_rows;
+    double *db = nullptr, *dx = nullptr;
+    if (hipMalloc((void **)&db, n * 8 + 8) != hipSuccess || hipMalloc((void **)&dx, n * 8 + 8) != hipSuccess) {
+        free_dev(db);
+        return MF_ERR_ALLOC;
+    }
+    mf_status st = MF_SUCCESS;
+    if (hipMemcpy(db, b, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dx, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        st = MF_ERR_HIP;
+    if (st == MF_SUCCESS) st = mf_bicgstab_solve(A, db, dx, opt, info, nullptr);
+    if (hipMemcpy(x, dx, n * 8, hipMemcpyDeviceToHost) != hipSuccess && st == MF_SUCCESS) st = MF_ERR_HIP;
+    free_dev(db);
+    free_dev(dx);
+    return st;
+}

@@ -1,0 +1,80 @@
+// Internal structures of the GPU solver library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "device_common.h"
+#include "mf_internal.h"
+
+namespace mf {
+
+struct Partition {
+    int G = 0;
+    int rows_cap = 0;
+    int *wg_tr = nullptr;           // device, G+1
+    long long *wg_chunk = nullptr;  // device, G+1
+};
+
+struct DeviceLayout {
+    int n_rows = 0, n_cols = 0, tilem = 0;
+    int64_t nnz = 0, tiles = 0, nchunks = 0;
+    std::vector<int64_t> tr_chunk;  // host, tilem+1
+    int4 *chunks = nullptr;
+    uint8_t *eidx = nullptr;
+    double *v64 = nullptr;
+    float *v32 = nullptr;
+    uint16_t *v16 = nullptr;
+    int8_t *v8 = nullptr;
+    bool has_lowp = false;
+    bool need_p64_mixed = false;
+    int64_t tiles_by_class[4] = {0, 0, 0, 0};
+    int64_t nnz_by_class[4] = {0, 0, 0, 0};
+    double alg_bytes_f64 = 0, alg_bytes_mixed = 0, device_bytes = 0;
+    std::map<int, Partition> parts;
+
+    DeviceLayout() = default;
+    DeviceLayout(const DeviceLayout &) = delete;
+    DeviceLayout &operator=(const DeviceLayout &) = delete;
+    ~DeviceLayout();
+    void release();
+    mf_status partition(int G, hipStream_t s, const Partition **out);
+    mfk::DevMatrix dev(const Partition &P) const;
+};
+
+// Per-matrix solver scratch (vectors, barrier records, status words).
+struct Workspace {
+    int64_t cap_n = 0;
+    int cap_g = 0;
+    double *p64 = nullptr, *v64 = nullptr, *s64 = nullptr, *rhat = nullptr;
+    float *p32 = nullptr, *v32 = nullptr, *s32 = nullptr;
+    unsigned long long *recs = nullptr;
+    unsigned *ctl = nullptr;
+    double *info = nullptr;
+    Workspace() = default;
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    ~Workspace();
+    mf_status ensure(int64_t n, int G, hipStream_t s);
+};
+
+mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s);
+void free_dev(void *p);
+int spmv_grid(const mf_matrix *A);
+
+}  // namespace mf
+
+struct mf_matrix {
+    int device = 0;
+    int class_policy = 0;
+    int num_cus = 0;
+    int max_lds = 65536;
+    mf::TileStruct ts;
+    std::vector<uint8_t> cls;
+    mf::DeviceLayout L;
+    mf::Workspace ws;
+    std::map<long long, int> tuned_grid;  // autotuner cache: key -> workgroups
+};
