@@ -239,8 +239,13 @@ mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr, const int 
                            mf_matrix **out);
 mf_status mf_matrix_destroy(mf_matrix *A);
 mf_status mf_matrix_get_stats(const mf_matrix *A, mf_matrix_stats *st);
-/* Overwrite the values (same pattern, CSR order), e.g. for finite differences. */
+/* Overwrite the values (same pattern, CSR order; host array), e.g. for finite
+ * differences: storage classes and value streams are rebuilt, the tiles are kept. */
 mf_status mf_matrix_update_values(mf_matrix *A, const double *val, void *stream);
+/* d_out[k] = scale * d_u[row(k)] * d_v[col(k)] for every stored entry k, written in the
+ * CSR order of the input (the value gradient -lambda_i x_j of an adjoint solve). */
+mf_status mf_pattern_outer(mf_matrix *A, const double *d_u, const double *d_v, double scale, double *d_out,
+                           void *stream);
 
 /* y = A x (or A^T x) on device pointers.  precision MF_PREC_MIXED applies the tile
  * classes (binary32 products, FP64 sums); MF_PREC_FP64 reads the FP64 stream. */
@@ -255,6 +260,11 @@ mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
  * main-bicg-mixed.cu:1355). */
 mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                             mf_solve_info *info, void *stream);
+/* Autotuner of the persistent CG launch: times short fixed-iteration solves over
+ * candidate workgroup counts and both variants, caches the fastest for this matrix
+ * and precision (used by later solves with grid = 0, variant = 0). */
+mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
+                         int *best_variant, double *best_ms_per_iter, void *stream);
 /* Host-vector conveniences (copy in, solve, copy out, synchronise). */
 mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,
                            mf_solve_info *info);
@@ -287,18 +297,6 @@ mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const
                               int use_graph, mf_solve_info *per_system, double *time_ms, void *stream);
 /* Workgroups per system chosen by the autotuner (written after a tuned solve). */
 mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
-
-/* ------------------------------------------------------------------------ */
-/* CPU baseline (OpenMP): tiled SpMV and host CG / BiCGSTAB loops, FP64.     */
-/* ------------------------------------------------------------------------ */
-mf_status mf_cpu_tile_spmv(int n_rows, int n_cols, const int *rowptr, const int *colidx,
-                           const double *val, const double *x, double *y, int threads);
-mf_status mf_cpu_cg(int n_rows, const int *rowptr, const int *colidx, const double *val,
-                    const double *b, double *x, double tol, int max_iter, int threads,
-                    int *iterations, double *relres);
-mf_status mf_cpu_bicgstab(int n_rows, const int *rowptr, const int *colidx, const double *val,
-                          const double *b, double *x, double tol, int max_iter, int threads,
-                          int *iterations, double *relres);
 
 /* Library / device information. */
 int mf_device_count(void);

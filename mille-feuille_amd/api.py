@@ -272,6 +272,19 @@ class TiledMatrix:
                               _PREC[precision], int(bool(transpose)), _stream_handle()), "mf_spmv")
         return y
 
+    def pattern_outer(self, u, v, scale=1.0, out=None):
+        """out[k] = scale * u[row(k)] * v[col(k)] over the stored entries, CSR order."""
+        torch = _torch()
+        dev = torch.device("cuda", self.device)
+        ut = _dev_vec(u, self.n, dev)
+        vt = _dev_vec(v, self.n_cols, dev)
+        nnz = int(self.stats()["nnz"])
+        o = out if out is not None else torch.empty(nnz, dtype=torch.float64, device=dev)
+        check(L.lib().mf_pattern_outer(self.h, ctypes.c_void_p(ut.data_ptr()), ctypes.c_void_p(vt.data_ptr()),
+                                       float(scale), ctypes.c_void_p(o.data_ptr()), _stream_handle()),
+              "mf_pattern_outer")
+        return o
+
     def _solve(self, fn_name, b, x0, opts):
         torch = _torch()
         dev = torch.device("cuda", self.device)
@@ -289,6 +302,19 @@ class TiledMatrix:
         if st not in (L.MF_SUCCESS, 4):  # MF_NOT_CONVERGED is reported, not raised
             raise MfError(st, fn_name)
         return xt, d
+
+    def autotune(self, b, precision="fp64"):
+        """Pick workgroups and variant of the persistent CG for this matrix; returns
+        (grid, variant, ms per iteration)."""
+        torch = _torch()
+        dev = torch.device("cuda", self.device)
+        bt = _dev_vec(b, self.n, dev)
+        xt = torch.zeros(self.n, dtype=torch.float64, device=dev)
+        g, v, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(L.lib().mf_cg_autotune(self.h, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                                     _PREC[precision], ctypes.byref(g), ctypes.byref(v), ctypes.byref(t),
+                                     _stream_handle()), "mf_cg_autotune")
+        return g.value, v.value, t.value
 
     def cg(self, b, x0=None, **kw):
         """Single-kernel CG; returns (x tensor, info dict)."""
@@ -309,3 +335,73 @@ def device_name(device=0):
     buf = ctypes.create_string_buffer(256)
     check(L.lib().mf_device_name(int(device), buf, 256), "mf_device_name")
     return buf.value.decode()
+
+
+class Batch:
+    """Many independent square systems solved by one persistent launch."""
+
+    def __init__(self, systems, class_policy="lossless", device=0, keep_mixed=True):
+        """systems: list of (rowptr, colidx, val) CSR triples."""
+        self._keep = [(as_i32(r), as_i32(c), as_f64(v)) for r, c, v in systems]
+        cnt = len(self._keep)
+        self.count = cnt
+        self.device = int(device)
+        rows = np.array([r.size - 1 for r, _, _ in self._keep], np.int32)
+        RP = (i32p * cnt)(*[ptr(r, i32p) for r, _, _ in self._keep])
+        CI = (i32p * cnt)(*[ptr(c, i32p) for _, c, _ in self._keep])
+        VA = (f64p * cnt)(*[ptr(v, f64p) for _, _, v in self._keep])
+        mo = L.MatrixOptions()
+        L.lib().mf_matrix_options_default(ctypes.byref(mo))
+        mo.class_policy = _POLICY[class_policy] if isinstance(class_policy, str) else int(class_policy)
+        mo.device = self.device
+        mo.keep_mixed = int(bool(keep_mixed))
+        h = ctypes.c_void_p()
+        _torch().cuda.set_device(self.device)
+        check(L.lib().mf_batch_create(cnt, ptr(rows, i32p), RP, CI, VA, ctypes.byref(mo), _stream_handle(),
+                                      ctypes.byref(h)), "mf_batch_create")
+        self.h = h
+        off = np.zeros(cnt + 1, np.int64)
+        check(L.lib().mf_batch_row_offsets(self.h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+              "mf_batch_row_offsets")
+        self.offsets = off
+        self.rows = rows
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().mf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rows_per_workgroup(self, r):
+        check(L.lib().mf_batch_set_groups(self.h, int(r)), "mf_batch_set_groups")
+
+    def pack(self, vectors):
+        """Concatenate per-system vectors at the padded offsets (zeros in the padding)."""
+        torch = _torch()
+        out = np.zeros(int(self.offsets[-1]))
+        for s, v in enumerate(vectors):
+            out[self.offsets[s]: self.offsets[s] + self.rows[s]] = np.asarray(v, np.float64)
+        return torch.as_tensor(out, device=torch.device("cuda", self.device))
+
+    def unpack(self, t):
+        a = t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+        return [a[self.offsets[s]: self.offsets[s] + self.rows[s]].copy() for s in range(self.count)]
+
+    def cg(self, b_packed, use_graph=True, **kw):
+        """Solve every system; returns (x packed tensor, list of info dicts, launch ms)."""
+        torch = _torch()
+        opts = solve_options(**kw)
+        x = torch.zeros_like(b_packed)
+        infos = (L.SolveInfo * self.count)()
+        ms = ctypes.c_double(0)
+        st = L.lib().mf_cg_solve_batched(self.h, ctypes.c_void_p(b_packed.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                          ctypes.byref(opts), int(bool(use_graph)), infos, ctypes.byref(ms),
+                                          _stream_handle())
+        if st not in (L.MF_SUCCESS, 4):
+            raise MfError(st, "mf_cg_solve_batched")
+        return x, [infos[i].as_dict() for i in range(self.count)], ms.value

@@ -1,0 +1,115 @@
+"""Differentiable sparse solve x = A(vals)^{-1} b over the C-ABI.
+
+``SparseSolve.apply(vals, b, op)`` returns x.  Forward runs the single-kernel CG
+(SPD) or BiCGSTAB (general) of libmf.so and stores only x.  Backward solves the
+adjoint system A^T lambda = g with the same kernel -- reusing A itself for SPD CG,
+a transposed tiled matrix for BiCGSTAB -- and returns
+
+    grad_b    = lambda
+    grad_vals = -lambda_i * x_j   on the sparsity pattern (CSR order), written by a
+                                  HIP kernel of the library (mf_pattern_outer).
+
+Both solves run to the operator's tolerance (default 1e-10 relative residual).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from .api import TiledMatrix
+
+
+class SparseOperator:
+    """A fixed sparsity pattern whose values may change between calls."""
+
+    def __init__(self, rowptr, colidx, vals, spd=True, precision="mixed", class_policy="rowscaled", tol=1e-10,
+                 max_iter=100000, device=0):
+        self.rowptr = np.ascontiguousarray(rowptr, np.int32)
+        self.colidx = np.ascontiguousarray(colidx, np.int32)
+        self.n = self.rowptr.size - 1
+        self.nnz = self.colidx.size
+        self.spd = bool(spd)
+        self.precision = precision
+        self.tol = float(tol)
+        self.max_iter = int(max_iter)
+        self.device = int(device)
+        self.class_policy = class_policy
+        v = np.ascontiguousarray(vals, np.float64)
+        self.A = TiledMatrix(self.rowptr, self.colidx, v, class_policy=class_policy, device=device)
+        self._vals = v.copy()
+        self.AT = None
+        if not self.spd:
+            # transposed pattern and the permutation from its CSR order to ours
+            rows = np.repeat(np.arange(self.n), np.diff(self.rowptr))
+            order = np.lexsort((rows, self.colidx))
+            self.t_perm = order
+            self.t_rowptr = np.zeros(self.n + 1, np.int64)
+            np.add.at(self.t_rowptr, self.colidx + 1, 1)
+            self.t_rowptr = np.cumsum(self.t_rowptr).astype(np.int32)
+            self.t_colidx = rows[order].astype(np.int32)
+            self.AT = TiledMatrix(self.t_rowptr, self.t_colidx, v[order], class_policy=class_policy, device=device)
+        self.last_info = {}
+
+    def set_values(self, vals):
+        v = vals.detach().to("cpu", dtype=__import__("torch").float64).numpy() if hasattr(vals, "detach") \
+            else np.asarray(vals, np.float64)
+        if v.shape != (self.nnz,):
+            raise ValueError("values do not match the pattern")
+        if np.array_equal(v, self._vals):
+            return
+        self._vals = v.copy()
+        self.A.update_values(v)
+        if self.AT is not None:
+            self.AT.update_values(v[self.t_perm])
+
+    def solve(self, b, transpose=False):
+        M = self.AT if (transpose and not self.spd) else self.A
+        kw = dict(tol=self.tol, max_iter=self.max_iter, precision=self.precision)
+        x, info = (M.cg(b, **kw) if self.spd else M.bicgstab(b, **kw))
+        if info["status"] != 0:
+            raise L.MfError(info["status"], "adjoint solve" if transpose else "solve")
+        self.last_info["adjoint" if transpose else "forward"] = info
+        return x
+
+    def grad_values(self, lam, x):
+        return self.A.pattern_outer(lam, x, scale=-1.0)
+
+    def close(self):
+        self.A.close()
+        if self.AT is not None:
+            self.AT.close()
+
+
+def _make_function():
+    import torch
+
+    class SparseSolve(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, vals, b, op):
+            op.set_values(vals)
+            x = op.solve(b)
+            ctx.op = op
+            ctx.save_for_backward(x)
+            return x
+
+        @staticmethod
+        def backward(ctx, g):
+            (x,) = ctx.saved_tensors
+            op = ctx.op
+            lam = op.solve(g.contiguous(), transpose=True)
+            grad_vals = op.grad_values(lam, x) if ctx.needs_input_grad[0] else None
+            grad_b = lam if ctx.needs_input_grad[1] else None
+            return grad_vals, grad_b, None
+
+    return SparseSolve
+
+
+_FN = None
+
+
+def sparse_solve(vals, b, op):
+    """x = A(vals)^{-1} b, differentiable in vals and b."""
+    global _FN
+    if _FN is None:
+        _FN = _make_function()
+    return _FN.apply(vals, b, op)

@@ -1,0 +1,498 @@
+// Batched single-kernel CG: many independent SPD systems of uneven sizes in one
+// persistent launch.
+//
+// The systems are laid out as one block-diagonal tiled matrix, every system's rows
+// starting at a multiple of 16 so that a tile row belongs to one system.  Systems are
+// binned by size into pools; a pool's workgroups form barrier groups of W (1, 2, 4
+// or 8) workgroups, W the smallest power of two that keeps a member at or below the
+// tuned rows-per-workgroup bound.  Each group pops systems from its pool's queue
+// (largest first) and solves them one after another with the CG core: W = 1 groups
+// synchronise with block barriers only, larger groups with flagged records among
+// their members.  A converged system retires its group to the next one in the same
+// launch.  The launch (queue reset + kernel) can be captured once into a hipGraph
+// and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "cg_core.h"
+#include "launch_common.h"
+#include "mf_internal.h"
+#include "solver_internal.h"
+
+namespace mfk {
+
+struct BatchArgs {
+    DevMatrix A;
+    const long long *tr_chunk;  // tilem+1
+    const int *sys_tr;          // count+1 tile-row starts
+    const int *sys_rows;        // count
+    const int *order;           // systems of every pool, largest first
+    const int *pool_W;          // npools
+    const int *pool_wg0;        // npools+1
+    const int *pool_sys0;       // npools+1 ranges into order
+    unsigned *queue;            // npools counters
+    int npools;
+    unsigned long long *recs;   // 16 u64 per workgroup
+    unsigned *ctl;
+    long long timeout;
+    const double *b;
+    double *x;
+    double *p64;
+    float *p32;
+    double *info;               // 8 doubles per system
+    double tol2, delta2;
+    int maxit, fixed, need64, need32, max_repl;
+};
+
+__global__ void batch_reset_kernel(unsigned long long *recs, long long n, unsigned *ctl, unsigned *queue, int npools)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        recs[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < 64) ctl[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < npools) queue[threadIdx.x] = 0u;
+}
+
+template <int RPT, bool MIXED, int U, bool SINGLE>
+__device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, int g0, double *scratch, double *q,
+                             double *xs)
+{
+    __shared__ int s_idx;
+    const int nsys = a.pool_sys0[pool + 1] - a.pool_sys0[pool];
+    GridOps gsy;
+    gsy.gs.recs = a.recs + (size_t)g0 * 2 * kMaxRed * 2;
+    gsy.gs.ctl = a.ctl;
+    gsy.gs.timeout = a.timeout;
+    gsy.gs.G = W;
+    gsy.wg = member;
+    gsy.epoch = 0;
+    BlockOps bsy;
+    for (;;) {
+        int idx;
+        if (SINGLE) {
+            if (threadIdx.x == 0) s_idx = (int)atomicAdd(&a.queue[pool], 1u);
+            __syncthreads();
+            idx = s_idx;
+            __syncthreads();
+        } else {
+            double v[1] = {0.0};
+            if (member == 0 && threadIdx.x == 0) v[0] = (double)atomicAdd(&a.queue[pool], 1u);
+            if (!gsy.template sum<1>(v, false, scratch)) return;
+            idx = (int)v[0];
+        }
+        if (idx >= nsys) return;
+        const int s = a.order[a.pool_sys0[pool] + idx];
+        const int t0 = a.sys_tr[s], t1 = a.sys_tr[s + 1];
+        const int mt0 = t0 + (int)((long long)(t1 - t0) * member / W);
+        const int mt1 = t0 + (int)((long long)(t1 - t0) * (member + 1) / W);
+        Rows<RPT, 1024> R;
+        R.row0 = mt0 * kTile;
+        R.nrows = max(0, min(mt1 * kTile, t0 * kTile + a.sys_rows[s]) - R.row0);
+        R.refresh();
+        CgPart part;
+        part.A = a.A;
+        part.c0 = a.tr_chunk[mt0];
+        part.c1 = a.tr_chunk[mt1];
+        part.b = a.b;
+        part.x = a.x;
+        part.p64 = a.p64;
+        part.p32 = a.p32;
+        part.tol2 = a.tol2;
+        part.delta2 = a.delta2;
+        part.maxit = a.maxit;
+        part.fixed = a.fixed;
+        part.use_x0 = 0;
+        part.need64 = a.need64;
+        part.need32 = a.need32;
+        part.max_repl = a.max_repl;
+        CgResult res;
+        if (SINGLE) res = cg_core<RPT, 2, MIXED, 1024, BlockOps, U>(part, R, bsy, scratch, q, xs);
+        else res = cg_core<RPT, 2, MIXED, 1024, GridOps, U>(part, R, gsy, scratch, q, xs);
+        if (member == 0 && threadIdx.x == 0) {
+            double *inf = a.info + (size_t)s * 8;
+            inf[0] = (double)res.it;
+            inf[1] = res.rr;
+            inf[2] = res.bb;
+            inf[3] = res.rt;
+            inf[4] = (double)res.status;
+            inf[5] = (double)res.repl;
+        }
+        if (res.status == ST_TIMEOUT) return;
+        __syncthreads();
+    }
+}
+
+template <int RPT, bool MIXED, int U>
+__global__ __launch_bounds__(1024) void batched_cg_kernel(BatchArgs a)
+{
+    extern __shared__ double smem[];
+    double *scratch = smem;
+    double *q = smem + 64;
+    double *xs = smem + 64 + a.A.rows_cap;
+    const int b = blockIdx.x;
+    int pool = 0;
+    while (pool + 1 < a.npools && b >= a.pool_wg0[pool + 1]) pool++;
+    if (b < a.pool_wg0[pool] || b >= a.pool_wg0[pool + 1]) return;
+    const int W = a.pool_W[pool];
+    const int local = b - a.pool_wg0[pool];
+    const int member = local % W;
+    const int g0 = b - member;
+    if (W == 1) batch_worker<RPT, MIXED, U, true>(a, pool, W, member, g0, scratch, q, xs);
+    else batch_worker<RPT, MIXED, U, false>(a, pool, W, member, g0, scratch, q, xs);
+}
+
+}  // namespace mfk
+
+struct mf_batch {
+    int count = 0;
+    mf_matrix *M = nullptr;            // block-diagonal matrix of all systems
+    std::vector<int> rows, tr0;        // per system
+    std::vector<int64_t> row_off;      // count+1 (padded offsets)
+    std::vector<int64_t> nnz;
+    int rows_per_wg = 8192;            // tuned bound on a member's rows
+    int unroll = 2;                    // chunks in flight per 16-lane slot
+    // device tables
+    int *d_sys_tr = nullptr, *d_sys_rows = nullptr, *d_order = nullptr;
+    int *d_pool_W = nullptr, *d_pool_wg0 = nullptr, *d_pool_sys0 = nullptr;
+    long long *d_tr_chunk = nullptr;
+    unsigned *d_queue = nullptr;
+    double *d_info = nullptr;
+    int npools = 0, G = 0;
+    bool planned = false;
+    // captured launch
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    long long graph_key = -1;
+    ~mf_batch();
+    void free_plan();
+};
+
+void mf_batch::free_plan()
+{
+    for (void *p : {(void *)d_order, (void *)d_pool_W, (void *)d_pool_wg0, (void *)d_pool_sys0})
+        mf::free_dev(p);
+    d_order = d_pool_W = d_pool_wg0 = d_pool_sys0 = nullptr;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_key = -1;
+    planned = false;
+}
+
+mf_batch::~mf_batch()
+{
+    free_plan();
+    for (void *p : {(void *)d_sys_tr, (void *)d_sys_rows, (void *)d_tr_chunk, (void *)d_queue, (void *)d_info})
+        mf::free_dev(p);
+    if (M) mf_matrix_destroy(M);
+}
+
+using namespace mf;
+
+extern "C" mf_status mf_batch_create(int count, const int *n_rows, const int *const *rowptr,
+                                     const int *const *colidx, const double *const *val,
+                                     const mf_matrix_options *opt, void *stream, mf_batch **out)
+{
+    if (!out || count <= 0 || !n_rows || !rowptr || !colidx || !val) return MF_ERR_INVALID;
+    *out = nullptr;
+    mf_batch *B = new mf_batch();
+    B->count = count;
+    B->rows.assign(n_rows, n_rows + count);
+    B->row_off.assign(count + 1, 0);
+    B->nnz.assign(count, 0);
+    for (int s = 0; s < count; s++) {
+        if (n_rows[s] <= 0) {
+            delete B;
+            return MF_ERR_INVALID;
+        }
+        B->row_off[s + 1] = B->row_off[s] + ((int64_t)n_rows[s] + 15) / 16 * 16;
+        B->nnz[s] = rowptr[s][n_rows[s]] - rowptr[s][0];
+    }
+    const int64_t N = B->row_off[count];
+    int64_t Z = 0;
+    for (int s = 0; s < count; s++) Z += B->nnz[s];
+    if (N >= INT32_MAX || Z >= INT32_MAX) {
+        delete B;
+        return MF_ERR_TOO_LARGE;
+    }
+    std::vector<int> rp(N + 1, 0), ci(Z);
+    std::vector<double> va(Z);
+    int64_t z = 0;
+    for (int s = 0; s < count; s++) {
+        const int64_t o = B->row_off[s];
+        for (int i = 0; i < n_rows[s]; i++) {
+            rp[o + i] = (int)z;
+            for (int j = rowptr[s][i]; j < rowptr[s][i + 1]; j++, z++) {
+                const int c = colidx[s][j];
+                if (c < 0 || c >= n_rows[s]) {
+                    delete B;
+                    return MF_ERR_INVALID;
+                }
+                ci[z] = (int)(o + c);
+                va[z] = val[s][j];
+            }
+        }
+        for (int64_t i = o + n_rows[s]; i < B->row_off[s + 1]; i++) rp[i] = (int)z;
+    }
+    rp[N] = (int)z;
+    mf_status st = mf_matrix_create((int)N, (int)N, rp.data(), ci.data(), va.data(), opt, stream, &B->M);
+    if (st != MF_SUCCESS) {
+        delete B;
+        return st;
+    }
+    std::vector<int> tr(count + 1);
+    for (int s = 0; s <= count; s++) tr[s] = (int)(B->row_off[s] / 16);
+    hipStream_t hs = (hipStream_t)stream;
+    if ((st = upload((void **)&B->d_sys_tr, tr.data(), sizeof(int) * (count + 1), hs)) != MF_SUCCESS ||
+        (st = upload((void **)&B->d_sys_rows, B->rows.data(), sizeof(int) * count, hs)) != MF_SUCCESS ||
+        (st = upload((void **)&B->d_tr_chunk, B->M->L.tr_chunk.data(), sizeof(long long) * (B->M->L.tilem + 1), hs)) !=
+            MF_SUCCESS) {
+        delete B;
+        return st;
+    }
+    if (hipMalloc((void **)&B->d_queue, 256) != hipSuccess ||
+        hipMalloc((void **)&B->d_info, sizeof(double) * 8 * count) != hipSuccess) {
+        delete B;
+        return MF_ERR_ALLOC;
+    }
+    if (hipStreamSynchronize(hs) != hipSuccess) {
+        delete B;
+        return MF_ERR_HIP;
+    }
+    B->tr0.assign(tr.begin(), tr.end() - 1);
+    *out = B;
+    return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_batch_destroy(mf_batch *B)
+{
+    if (!B) return MF_ERR_INVALID;
+    delete B;
+    return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_batch_row_offsets(const mf_batch *B, int64_t *offsets)
+{
+    if (!B || !offsets) return MF_ERR_INVALID;
+    for (int s = 0; s <= B->count; s++) offsets[s] = B->row_off[s];
+    return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_batch_set_groups(mf_batch *B, int rows_per_wg_max)
+{
+    if (!B) return MF_ERR_INVALID;
+    int r = rows_per_wg_max;
+    if (r != 1024 && r != 2048 && r != 4096 && r != 8192) return MF_ERR_INVALID;
+    if (r != B->rows_per_wg) B->free_plan();
+    B->rows_per_wg = r;
+    return MF_SUCCESS;
+}
+
+namespace mf {
+
+// Size classes -> pools of workgroups proportional to each class's nonzeros.
+static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
+{
+    if (B->planned) return MF_SUCCESS;
+    const int R = B->rows_per_wg;
+    std::vector<std::vector<int>> cls(4);
+    for (int sidx = 0; sidx < B->count; sidx++) {
+        const int tiles_rows = (int)((B->row_off[sidx + 1] - B->row_off[sidx]) / 16);
+        int W = 1, ci = 0;
+        while (ci < 3 && (long long)(tiles_rows + W - 1) / W * 16 > R) {
+            W *= 2;
+            ci++;
+        }
+        if ((long long)(tiles_rows + W - 1) / W * 16 > R) return MF_ERR_TOO_LARGE;
+        cls[ci].push_back(sidx);
+    }
+    std::vector<int> pW, pwg0, psys0, order;
+    std::vector<double> work;
+    double total = 0;
+    for (int ci = 0; ci < 4; ci++) {
+        if (cls[ci].empty()) continue;
+        auto &v = cls[ci];
+        std::sort(v.begin(), v.end(), [&](int a, int b) { return B->nnz[a] > B->nnz[b]; });
+        double w = 0;
+        for (int x : v) w += (double)B->nnz[x] + (double)B->rows[x];
+        pW.push_back(1 << ci);
+        work.push_back(w);
+        total += w;
+        psys0.push_back((int)order.size());
+        order.insert(order.end(), v.begin(), v.end());
+    }
+    psys0.push_back((int)order.size());
+    const int np = (int)pW.size();
+    // workgroups per pool: proportional to work, multiple of W, at least W, at most
+    // W * (number of systems in the pool)
+    std::vector<int> nwg(np);
+    int used = 0;
+    for (int p = 0; p < np; p++) {
+        const int W = pW[p];
+        const int nsys = psys0[p + 1] - psys0[p];
+        int g = (int)std::floor(Gmax * work[p] / total / W) * W;
+        g = std::max(W, std::min(g, W * nsys));
+        nwg[p] = g;
+        used += g;
+    }
+    while (used > Gmax) {  // trim the largest pools
+        int p = (int)(std::max_element(nwg.begin(), nwg.end()) - nwg.begin());
+        if (nwg[p] <= pW[p]) return MF_ERR_OCCUPANCY;
+        nwg[p] -= pW[p];
+        used -= pW[p];
+    }
+    pwg0.push_back(0);
+    for (int p = 0; p < np; p++) pwg0.push_back(pwg0.back() + nwg[p]);
+    B->G = pwg0.back();
+    B->npools = np;
+    mf_status st;
+    if ((st = upload((void **)&B->d_order, order.data(), sizeof(int) * order.size(), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&B->d_pool_W, pW.data(), sizeof(int) * np, s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&B->d_pool_wg0, pwg0.data(), sizeof(int) * (np + 1), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&B->d_pool_sys0, psys0.data(), sizeof(int) * (np + 1), s)) != MF_SUCCESS) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
+    B->planned = true;
+    return MF_SUCCESS;
+}
+
+using BatchKernel = void (*)(mfk::BatchArgs);
+
+template <bool MIXED, int U>
+static BatchKernel pick_batch(int rpt)
+{
+    switch (rpt) {
+    case 1: return mfk::batched_cg_kernel<1, MIXED, U>;
+    case 2: return mfk::batched_cg_kernel<2, MIXED, U>;
+    case 4: return mfk::batched_cg_kernel<4, MIXED, U>;
+    case 8: return mfk::batched_cg_kernel<8, MIXED, U>;
+    default: return nullptr;
+    }
+}
+
+}  // namespace mf
+
+extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
+                                         int use_graph, mf_solve_info *per_system, double *time_ms, void *stream)
+{
+    if (!B || !d_b || !d_x) return MF_ERR_INVALID;
+    mf_matrix *A = B->M;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    mf_solve_options o;
+    mf_solve_options_default(&o);
+    if (opt) o = *opt;
+    const bool mixed = o.precision == MF_PREC_MIXED && A->L.has_lowp;
+    const int rpt = B->rows_per_wg / 1024;
+    BatchKernel k = B->unroll == 4 ? (mixed ? pick_batch<true, 4>(rpt) : pick_batch<false, 4>(rpt))
+                                   : (mixed ? pick_batch<true, 2>(rpt) : pick_batch<false, 2>(rpt));
+    if (!k) return MF_ERR_INVALID;
+    const size_t lds = (64 + 2 * (size_t)B->rows_per_wg) * sizeof(double);
+    if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
+    if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return MF_ERR_HIP;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 1024, lds) != hipSuccess) return MF_ERR_HIP;
+    const int Gmax = std::min(occ * A->num_cus, o.grid > 0 ? o.grid : occ * A->num_cus);
+    mf_status st = batch_plan(B, Gmax, s);
+    if (st != MF_SUCCESS) return st;
+    if (B->G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
+    if ((st = A->ws.ensure(A->L.n_rows, B->G, s)) != MF_SUCCESS) return st;
+    mfk::BatchArgs a;
+    Partition P0;  // the batched kernel uses the tables below, not a persistent partition
+    P0.rows_cap = B->rows_per_wg;
+    a.A = A->L.dev(P0);
+    a.tr_chunk = B->d_tr_chunk;
+    a.sys_tr = B->d_sys_tr;
+    a.sys_rows = B->d_sys_rows;
+    a.order = B->d_order;
+    a.pool_W = B->d_pool_W;
+    a.pool_wg0 = B->d_pool_wg0;
+    a.pool_sys0 = B->d_pool_sys0;
+    a.queue = B->d_queue;
+    a.npools = B->npools;
+    a.recs = A->ws.recs;
+    a.ctl = A->ws.ctl;
+    a.timeout = watchdog_cycles(o.watchdog_ms);
+    a.b = d_b;
+    a.x = d_x;
+    a.p64 = A->ws.p64;
+    a.p32 = A->ws.p32;
+    a.info = B->d_info;
+    a.tol2 = o.tol * o.tol;
+    const double dl = o.replace_delta > 0 ? o.replace_delta : 1e-3;
+    a.delta2 = dl * dl;
+    a.maxit = o.max_iter;
+    a.fixed = o.fixed_iterations;
+    a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
+    a.need32 = mixed ? 1 : 0;
+    a.max_repl = o.max_replacements > 0 ? o.max_replacements : 64;
+    const long long nrec = 2LL * 2 * mfk::kMaxRed * B->G;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MF_ERR_HIP;
+    const long long key = ((long long)(mixed ? 1 : 0) << 40) ^ (long long)(intptr_t)d_b ^ ((long long)(intptr_t)d_x << 1) ^
+                          ((long long)o.max_iter << 20) ^ (long long)(o.tol * 1e15);
+    (void)hipEventRecord(e0, s);
+    if (use_graph) {
+        if (B->graph_key != key) {
+            if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+            if (B->graph) (void)hipGraphDestroy(B->graph);
+            B->gexec = nullptr;
+            B->graph = nullptr;
+            hipStream_t cs;
+            if (hipStreamCreate(&cs) != hipSuccess) return MF_ERR_HIP;
+            if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return MF_ERR_HIP;
+            hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, cs, A->ws.recs, nrec, A->ws.ctl,
+                               B->d_queue, B->npools);
+            hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, cs, a);
+            if (hipStreamEndCapture(cs, &B->graph) != hipSuccess) return MF_ERR_HIP;
+            (void)hipStreamDestroy(cs);
+            if (hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0) != hipSuccess) return MF_ERR_HIP;
+            B->graph_key = key;
+            (void)hipEventRecord(e0, s);
+        }
+        if (hipGraphLaunch(B->gexec, s) != hipSuccess) return MF_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl,
+                           B->d_queue, B->npools);
+        hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, s, a);
+        if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+    }
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) return MF_ERR_HIP;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (time_ms) *time_ms = ms;
+    std::vector<double> h((size_t)8 * B->count);
+    if (hipMemcpy(h.data(), B->d_info, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return MF_ERR_HIP;
+    int worst = MF_SUCCESS;
+    for (int sidx = 0; sidx < B->count; sidx++) {
+        const double *f = h.data() + (size_t)8 * sidx;
+        const int status = (int)f[4];
+        if (status != MF_SUCCESS && worst == MF_SUCCESS) worst = status;
+        if (status == MF_TIMEOUT) worst = MF_TIMEOUT;
+        if (per_system) {
+            mf_solve_info &in = per_system[sidx];
+            memset(&in, 0, sizeof(in));
+            in.status = status;
+            in.iterations = (int)f[0];
+            in.replacements = (int)f[5];
+            const double bb = f[2];
+            in.relres = bb > 0 ? std::sqrt(std::max(f[1], 0.0) / bb) : 0.0;
+            in.relres_true = bb > 0 && f[3] >= 0 ? std::sqrt(f[3] / bb) : (f[3] >= 0 ? std::sqrt(f[3]) : -1.0);
+            in.time_ms = ms;
+            in.grid = B->G;
+            in.block = 1024;
+            in.rows_per_thread = rpt;
+        }
+    }
+    return (mf_status)worst;
+}

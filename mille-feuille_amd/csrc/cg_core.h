@@ -1,0 +1,341 @@
+// CG core shared by the single-system and the batched persistent kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace mfk {
+
+struct CgArgs {
+    DevMatrix A;
+    GridSync gs;
+    const double *b;
+    double *x;       // in: x0 when use_x0; out: solution
+    double *p64;
+    float *p32;
+    double *info;    // [0] iterations [1] r'r [2] b'b [3] true r'r [4] status [5] replacements
+    double tol2;     // tol^2
+    double delta2;   // replacement trigger (squared ratio)
+    int maxit;
+    int fixed;
+    int use_x0;
+    int need64;
+    int need32;
+    int max_repl;
+};
+
+enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
+
+// Synchronisation policies of the CG core.
+//   GridOps : all workgroups of the launch (flagged-record grid all-reduce)
+//   BlockOps: one workgroup owns the whole system (block reduction + barrier)
+struct GridOps {
+    GridSync gs;
+    int wg;
+    unsigned epoch;
+    template <int K>
+    __device__ __forceinline__ bool sum(double (&v)[K], bool visibility, double *scratch)
+    {
+        return grid_sum<K>(gs, v, epoch, visibility, wg, scratch);
+    }
+};
+
+struct BlockOps {
+    template <int K>
+    __device__ __forceinline__ bool sum(double (&v)[K], bool visibility, double *scratch)
+    {
+        if (visibility) wait_own_stores();
+        block_sum<K>(v, scratch);
+        return true;
+    }
+};
+
+// Rows [row0, row0+nrows) of one system held by this workgroup, its chunk range and
+// vectors.
+struct CgPart {
+    DevMatrix A;
+    long long c0, c1;
+    const double *b;
+    double *x;
+    double *p64;
+    float *p32;
+    double tol2, delta2;
+    int maxit, fixed, use_x0, need64, need32, max_repl;
+};
+
+struct CgResult {
+    int it, repl, status;
+    double rr, bb, rt;
+};
+
+template <int RPT, int BLK>
+__device__ __forceinline__ void publish_p(const CgPart &a, Rows<RPT, BLK> &R, const double (&p)[RPT])
+{
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        if (a.need64) a.p64[row] = p[k];
+        if (a.need32) a.p32[row] = (float)p[k];
+    }
+}
+
+// r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old)
+// when fix_p; returns the r'r sum.  One SpMV and two syncs.
+template <int RPT, int BLK, class Sync>
+__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *q, double *scratch,
+                                 const double *xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
+{
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+    double dummy[1] = {0.0};
+    if (!sy.template sum<1>(dummy, true, scratch)) return false;
+    zero_q<RPT, BLK>(q, R);
+    __syncthreads();
+    spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, a.c0, a.c1, R.row0, a.x, nullptr, q);
+    __syncthreads();
+    double part[1] = {0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        if (!R.ok(k)) continue;
+        const double rt = a.b[R.row0 + R.loc(k)] - q[R.loc(k)];
+        if (fix_p) p[k] += rt - r[k];
+        r[k] = rt;
+        part[0] += rt * rt;
+    }
+    if (fix_p) publish_p<RPT, BLK>(a, R, p);
+    if (!sy.template sum<1>(part, true, scratch)) return false;
+    rr = part[0];
+    return true;
+}
+
+// The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
+// a.x; every member returns the same CgResult.
+template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = (RPT >= 8 ? 2 : 4)>
+__device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q, double *xs)
+{
+    CgResult res;
+    res.it = 0;
+    res.repl = 0;
+    res.status = ST_NOTCONV;
+    res.rr = res.bb = 0.0;
+    res.rt = -1.0;
+    double r[RPT], p[RPT];
+    // ---- setup: r = b - A x0, p = r ----
+    if (a.use_x0) {
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<false, U>(a.A, a.c0, a.c1, R.row0, a.x, nullptr, q);
+        __syncthreads();
+    }
+    double part[2] = {0.0, 0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        r[k] = p[k] = 0.0;
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        const double bk = a.b[row];
+        xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
+        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        p[k] = r[k];
+        part[0] += r[k] * r[k];
+        part[1] += bk * bk;
+    }
+    publish_p<RPT, BLK>(a, R, p);
+    double rr = 0.0, bb = 0.0, rbase = 0.0;
+    bool alive = sy.template sum<2>(part, true, scratch);
+    if (alive) {
+        rr = part[0];
+        bb = part[1];
+        rbase = rr;
+        if (bb == 0.0) {
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                if (R.ok(k)) xs[R.loc(k)] = 0.0;
+            rr = 0.0;
+            res.status = ST_OK;
+        }
+    } else {
+        res.status = ST_TIMEOUT;
+    }
+
+    while (alive && res.status == ST_NOTCONV) {
+        if (!a.fixed) {
+            bool conv = rr <= a.tol2 * bb;
+            if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
+                if (!replace_residual<RPT, BLK>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
+                    res.status = ST_TIMEOUT;
+                    break;
+                }
+                res.repl++;
+                rbase = rr;
+                conv = rr <= a.tol2 * bb;
+            }
+            if (conv) {
+                res.status = ST_OK;
+                break;
+            }
+        }
+        if (res.it >= a.maxit) {
+            res.status = a.fixed ? ST_OK : ST_NOTCONV;
+            break;
+        }
+        // q = A p
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<MIXED, U>(a.A, a.c0, a.c1, R.row0, a.p64, a.p32, q);
+        __syncthreads();
+        R.refresh();
+        if (VAR == 1) {
+            double d[1] = {0.0};
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                if (R.ok(k)) d[0] += p[k] * q[R.loc(k)];
+            if (!sy.template sum<1>(d, false, scratch)) {
+                res.status = ST_TIMEOUT;
+                break;
+            }
+            if (!(d[0] != 0.0) || !isfinite(d[0])) {
+                res.status = ST_BREAK;
+                break;
+            }
+            const double alpha = rr / d[0];
+            double e[1] = {0.0};
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                xs[R.loc(k)] += alpha * p[k];
+                r[k] -= alpha * q[R.loc(k)];
+                e[0] += r[k] * r[k];
+            }
+            if (!sy.template sum<1>(e, false, scratch)) {
+                res.status = ST_TIMEOUT;
+                break;
+            }
+            const double beta = e[0] / rr;
+            rr = e[0];
+#pragma unroll
+            for (int k = 0; k < RPT; k++) p[k] = r[k] + beta * p[k];
+            publish_p<RPT, BLK>(a, R, p);
+            double z[1] = {0.0};
+            if (!sy.template sum<1>(z, true, scratch)) {
+                res.status = ST_TIMEOUT;
+                break;
+            }
+        } else {
+            double d[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                const double qk = q[R.loc(k)];
+                d[0] += p[k] * qk;
+                d[1] += r[k] * qk;
+                d[2] += qk * qk;
+            }
+            if (!sy.template sum<3>(d, false, scratch)) {
+                res.status = ST_TIMEOUT;
+                break;
+            }
+            if (!(d[0] != 0.0) || !isfinite(d[0])) {
+                res.status = ST_BREAK;
+                break;
+            }
+            const double alpha = rr / d[0];
+            double rr_new = rr - 2.0 * alpha * d[1] + alpha * alpha * d[2];
+            if (!(rr_new > 0.0)) rr_new = 0.0;
+            const double beta = rr_new / rr;
+            double e[1] = {0.0};
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                xs[R.loc(k)] += alpha * p[k];
+                r[k] -= alpha * q[R.loc(k)];
+                p[k] = r[k] + beta * p[k];
+                e[0] += r[k] * r[k];
+            }
+            publish_p<RPT, BLK>(a, R, p);
+            if (!sy.template sum<1>(e, true, scratch)) {
+                res.status = ST_TIMEOUT;
+                break;
+            }
+            rr = e[0];
+        }
+        res.it++;
+    }
+    // write x, then the binary64 true residual ||b - A x||^2
+    if (res.status != ST_TIMEOUT) {
+        double rtrue[RPT], pd[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            rtrue[k] = r[k];
+            pd[k] = p[k];
+        }
+        double rr_t = 0.0;
+        if (replace_residual<RPT, BLK>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false)) res.rt = rr_t;
+        else res.status = ST_TIMEOUT;
+    } else {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+    }
+    res.rr = rr;
+    res.bb = bb;
+    return res;
+}
+
+template <int RPT, int VAR, bool MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
+{
+    extern __shared__ double smem[];
+    double *scratch = smem;                  // 64 doubles for reductions
+    double *q = smem + 64;                   // rows_cap doubles: SpMV result
+    double *xs = smem + 64 + a.A.rows_cap;   // rows_cap doubles: this workgroup's x
+    const int G = a.gs.G;
+    const int wg = logical_wg(blockIdx.x, G);
+    const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
+    Rows<RPT, BLK> R;
+    R.row0 = tr0 * kTile;
+    R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
+    R.refresh();
+    CgPart part;
+    part.A = a.A;
+    part.c0 = a.A.wg_chunk[wg];
+    part.c1 = a.A.wg_chunk[wg + 1];
+    part.b = a.b;
+    part.x = a.x;
+    part.p64 = a.p64;
+    part.p32 = a.p32;
+    part.tol2 = a.tol2;
+    part.delta2 = a.delta2;
+    part.maxit = a.maxit;
+    part.fixed = a.fixed;
+    part.use_x0 = a.use_x0;
+    part.need64 = a.need64;
+    part.need32 = a.need32;
+    part.max_repl = a.max_repl;
+    GridOps sy;
+    sy.gs = a.gs;
+    sy.wg = wg;
+    sy.epoch = 0;
+    CgResult res = cg_core<RPT, VAR, MIXED, BLK>(part, R, sy, scratch, q, xs);
+    if (wg == 0 && threadIdx.x == 0) {
+        a.info[0] = (double)res.it;
+        a.info[1] = res.rr;
+        a.info[2] = res.bb;
+        a.info[3] = res.rt;
+        a.info[4] = (double)res.status;
+        a.info[5] = (double)res.repl;
+    }
+}
+
+}  // namespace mfk

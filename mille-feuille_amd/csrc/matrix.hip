@@ -54,6 +54,7 @@ void DeviceLayout::release()
     free_dev(v32);
     free_dev(v16);
     free_dev(v8);
+    free_dev(src);
     for (auto &kv : parts) {
         free_dev(kv.second.wg_tr);
         free_dev(kv.second.wg_chunk);
@@ -65,6 +66,7 @@ void DeviceLayout::release()
     v32 = nullptr;
     v16 = nullptr;
     v8 = nullptr;
+    src = nullptr;
 }
 
 static mf_status build_layout(const TileStruct &ts, const double *val, const std::vector<uint8_t> &cls,
@@ -135,6 +137,11 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     if ((st = upload((void **)&L.chunks, ch.data(), sizeof(int4) * C, s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&L.eidx, eid.data(), ts.nnz, s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&L.v64, v64.data(), sizeof(double) * ts.nnz, s)) != MF_SUCCESS) return st;
+    {
+        std::vector<int> src32(ts.nnz > 0 ? ts.nnz : 1);
+        for (int64_t e = 0; e < ts.nnz; e++) src32[e] = (int)ts.src[e];
+        if ((st = upload((void **)&L.src, src32.data(), sizeof(int) * ts.nnz, s)) != MF_SUCCESS) return st;
+    }
     if (L.has_lowp) {
         if ((st = upload((void **)&L.v32, v32.data(), sizeof(float) * ts.nnz, s)) != MF_SUCCESS) return st;
         if ((st = upload((void **)&L.v16, v16.data(), sizeof(uint16_t) * ts.nnz, s)) != MF_SUCCESS) return st;
@@ -277,6 +284,21 @@ __global__ __launch_bounds__(1024) void spmv_kernel(DevMatrix A, const double *_
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
 }
 
+// out[src[e]] = scale * u[row(e)] * v[col(e)] for every stored entry (CSR order out)
+__global__ void pattern_outer_kernel(DevMatrix A, const int *__restrict__ src, long long nchunks,
+                                     const double *__restrict__ u, const double *__restrict__ v, double scale,
+                                     double *__restrict__ out, int base)
+{
+    const long long c = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 4;
+    const int k = threadIdx.x & 15;
+    if (c >= nchunks) return;
+    const int4 m = A.chunks[c];
+    if (k > (m.w & 15)) return;
+    const long long e = (long long)m.x + k;
+    const int id = A.eidx[e];
+    out[src[e] - base] = scale * u[m.z + (id >> 4)] * v[m.y + (id & 15)];
+}
+
 __global__ void to_f32_kernel(const double *__restrict__ x, float *__restrict__ y, int n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = (float)x[i];
@@ -308,6 +330,8 @@ extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr,
     mf_matrix *A = new mf_matrix();
     A->device = o.device;
     A->class_policy = o.class_policy;
+    A->keep_mixed = o.keep_mixed;
+    if (n_rows >= 0 && rowptr) A->rowptr.assign(rowptr, rowptr + n_rows + 1);
     mf_status st = build_tiles(n_rows, n_cols, rowptr, colidx, A->ts);
     if (st == MF_SUCCESS) {
         tile_classes(A->ts, rowptr, val, o.class_policy, A->cls);
@@ -326,6 +350,33 @@ extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr,
     }
     *out = A;
     return MF_SUCCESS;
+}
+
+extern "C" mf_status mf_matrix_update_values(mf_matrix *A, const double *val, void *stream)
+{
+    if (!A || (!val && A->ts.nnz > 0)) return MF_ERR_INVALID;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
+    tile_classes(A->ts, A->rowptr.data(), val, A->class_policy, A->cls);
+    A->L.release();
+    return build_layout(A->ts, val, A->cls, A->keep_mixed != 0, s, A->L);
+}
+
+extern "C" mf_status mf_pattern_outer(mf_matrix *A, const double *d_u, const double *d_v, double scale,
+                                      double *d_out, void *stream)
+{
+    if (!A || !d_u || !d_v || !d_out) return MF_ERR_INVALID;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    if (A->L.nchunks == 0) return MF_SUCCESS;
+    const Partition *P;
+    mf_status st = A->L.partition(1, (hipStream_t)stream, &P);
+    if (st != MF_SUCCESS) return st;
+    const long long threads = A->L.nchunks * 16;
+    const int base = A->rowptr.empty() ? 0 : A->rowptr[0];
+    hipLaunchKernelGGL(mfk::pattern_outer_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, A->L.dev(*P), A->L.src, A->L.nchunks, d_u, d_v, scale, d_out, base);
+    return hipGetLastError() == hipSuccess ? MF_SUCCESS : MF_ERR_HIP;
 }
 
 extern "C" mf_status mf_matrix_destroy(mf_matrix *A)

@@ -34,267 +34,9 @@
 #include "solver_internal.h"
 #include "launch_common.h"
 
+#include "cg_core.h"
+
 namespace mfk {
-
-struct CgArgs {
-    DevMatrix A;
-    GridSync gs;
-    const double *b;
-    double *x;       // in: x0 when use_x0; out: solution
-    double *p64;
-    float *p32;
-    double *info;    // [0] iterations [1] r'r [2] b'b [3] true r'r [4] status [5] replacements
-    double tol2;     // tol^2
-    double delta2;   // replacement trigger (squared ratio)
-    int maxit;
-    int fixed;
-    int use_x0;
-    int need64;
-    int need32;
-    int max_repl;
-};
-
-enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
-
-template <int RPT, int BLK>
-__device__ __forceinline__ void publish_p(const CgArgs &a, Rows<RPT, BLK> &R, const double (&p)[RPT])
-{
-    R.refresh();
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
-        if (a.need64) a.p64[row] = p[k];
-        if (a.need32) a.p32[row] = (float)p[k];
-    }
-}
-
-// r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old);
-// returns the grid r'r.  Costs one SpMV and two grid syncs.
-template <int RPT, int BLK>
-__device__ bool replace_residual(const CgArgs &a, Rows<RPT, BLK> &R, long long c0, long long c1, int wg,
-                                 double *q, double *scratch, unsigned &epoch, const double *xs, double (&r)[RPT],
-                                 double (&p)[RPT], double &rr, bool fix_p)
-{
-    R.refresh();
-#pragma unroll
-    for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
-    double dummy[1] = {0.0};
-    if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
-    zero_q<RPT, BLK>(q, R);
-    __syncthreads();
-    spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, c0, c1, R.row0, a.x, nullptr, q);
-    __syncthreads();
-    double part[1] = {0.0};
-    R.refresh();
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        if (!R.ok(k)) continue;
-        const double rt = a.b[R.row0 + R.loc(k)] - q[R.loc(k)];
-        if (fix_p) p[k] += rt - r[k];
-        r[k] = rt;
-        part[0] += rt * rt;
-    }
-    if (fix_p) publish_p<RPT, BLK>(a, R, p);
-    if (!grid_sum<1>(a.gs, part, epoch, true, wg, scratch)) return false;
-    rr = part[0];
-    return true;
-}
-
-template <int RPT, int VAR, bool MIXED, int BLK>
-__global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
-{
-    extern __shared__ double smem[];
-    double *scratch = smem;                  // 64 doubles for reductions
-    double *q = smem + 64;                   // rows_cap doubles: SpMV result
-    double *xs = smem + 64 + a.A.rows_cap;   // rows_cap doubles: this workgroup's x
-    const int G = a.gs.G;
-    const int wg = logical_wg(blockIdx.x, G);
-    const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
-    Rows<RPT, BLK> R;
-    R.row0 = tr0 * kTile;
-    R.refresh();
-    R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
-    const long long c0 = a.A.wg_chunk[wg], c1 = a.A.wg_chunk[wg + 1];
-    unsigned epoch = 0;
-
-    double r[RPT], p[RPT];
-    // ---- setup: r = b - A x0, p = r ----
-    if (a.use_x0) {
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, c0, c1, R.row0, a.x, nullptr, q);
-        __syncthreads();
-    }
-    double part[2] = {0.0, 0.0};
-    R.refresh();
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        r[k] = p[k] = 0.0;
-        if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
-        const double bk = a.b[row];
-        xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
-        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
-        p[k] = r[k];
-        part[0] += r[k] * r[k];
-        part[1] += bk * bk;
-    }
-    publish_p<RPT, BLK>(a, R, p);
-    int status = ST_NOTCONV;
-    int it = 0, repl = 0;
-    double rr = 0.0, bb = 0.0, rbase = 0.0;
-    if (!grid_sum<2>(a.gs, part, epoch, true, wg, scratch)) {
-        status = ST_TIMEOUT;
-        goto done;
-    }
-    rr = part[0];
-    bb = part[1];
-    rbase = rr;
-    if (bb == 0.0) {
-        R.refresh();
-#pragma unroll
-        for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) xs[R.loc(k)] = 0.0;
-        rr = 0.0;
-        status = ST_OK;
-        goto done;
-    }
-
-    for (;;) {
-        if (!a.fixed) {
-            bool conv = rr <= a.tol2 * bb;
-            if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                if (!replace_residual<RPT, BLK>(a, R, c0, c1, wg, q, scratch, epoch, xs, r, p, rr, true)) {
-                    status = ST_TIMEOUT;
-                    break;
-                }
-                repl++;
-                rbase = rr;
-                conv = rr <= a.tol2 * bb;
-            }
-            if (conv) {
-                status = ST_OK;
-                break;
-            }
-        }
-        if (it >= a.maxit) {
-            status = a.fixed ? ST_OK : ST_NOTCONV;
-            break;
-        }
-        // q = A p
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<MIXED, (RPT >= 8 ? 2 : 4)>(a.A, c0, c1, R.row0, a.p64, a.p32, q);
-        __syncthreads();
-        R.refresh();
-        if (VAR == 1) {
-            double d[1] = {0.0};
-#pragma unroll
-            for (int k = 0; k < RPT; k++)
-                if (R.ok(k)) d[0] += p[k] * q[R.loc(k)];
-            if (!grid_sum<1>(a.gs, d, epoch, false, wg, scratch)) {
-                status = ST_TIMEOUT;
-                break;
-            }
-            if (!(d[0] != 0.0) || !isfinite(d[0])) {
-                status = ST_BREAK;
-                break;
-            }
-            const double alpha = rr / d[0];
-            double e[1] = {0.0};
-            R.refresh();
-#pragma unroll
-            for (int k = 0; k < RPT; k++) {
-                if (!R.ok(k)) continue;
-                xs[R.loc(k)] += alpha * p[k];
-                r[k] -= alpha * q[R.loc(k)];
-                e[0] += r[k] * r[k];
-            }
-            if (!grid_sum<1>(a.gs, e, epoch, false, wg, scratch)) {
-                status = ST_TIMEOUT;
-                break;
-            }
-            const double beta = e[0] / rr;
-            rr = e[0];
-#pragma unroll
-            for (int k = 0; k < RPT; k++) p[k] = r[k] + beta * p[k];
-            publish_p<RPT, BLK>(a, R, p);
-            double z[1] = {0.0};
-            if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
-                status = ST_TIMEOUT;
-                break;
-            }
-        } else {
-            double d[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k = 0; k < RPT; k++) {
-                if (!R.ok(k)) continue;
-                const double qk = q[R.loc(k)];
-                d[0] += p[k] * qk;
-                d[1] += r[k] * qk;
-                d[2] += qk * qk;
-            }
-            if (!grid_sum<3>(a.gs, d, epoch, false, wg, scratch)) {
-                status = ST_TIMEOUT;
-                break;
-            }
-            if (!(d[0] != 0.0) || !isfinite(d[0])) {
-                status = ST_BREAK;
-                break;
-            }
-            const double alpha = rr / d[0];
-            double rr_new = rr - 2.0 * alpha * d[1] + alpha * alpha * d[2];
-            if (!(rr_new > 0.0)) rr_new = 0.0;
-            const double beta = rr_new / rr;
-            double e[1] = {0.0};
-            R.refresh();
-#pragma unroll
-            for (int k = 0; k < RPT; k++) {
-                if (!R.ok(k)) continue;
-                xs[R.loc(k)] += alpha * p[k];
-                r[k] -= alpha * q[R.loc(k)];
-                p[k] = r[k] + beta * p[k];
-                e[0] += r[k] * r[k];
-            }
-            publish_p<RPT, BLK>(a, R, p);
-            if (!grid_sum<1>(a.gs, e, epoch, true, wg, scratch)) {
-                status = ST_TIMEOUT;
-                break;
-            }
-            rr = e[0];
-        }
-        it++;
-    }
-done:
-    // write x, then the binary64 true residual ||b - A x||^2
-    double rt = -1.0;
-    if (status != ST_TIMEOUT) {
-        double rtrue[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) rtrue[k] = r[k];
-        double pd[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) pd[k] = p[k];
-        double rr_t = 0.0;
-        if (replace_residual<RPT, BLK>(a, R, c0, c1, wg, q, scratch, epoch, xs, rtrue, pd, rr_t, false)) rt = rr_t;
-        else status = ST_TIMEOUT;
-    } else {
-        R.refresh();
-#pragma unroll
-        for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
-    }
-    if (wg == 0 && threadIdx.x == 0) {
-        a.info[0] = (double)it;
-        a.info[1] = rr;
-        a.info[2] = bb;
-        a.info[3] = rt;
-        a.info[4] = (double)status;
-        a.info[5] = (double)repl;
-    }
-}
 
 __global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigned *ctl, double *info)
 {
@@ -365,7 +107,8 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
                     hipStream_t s)
 {
     const bool mixed = o->precision == MF_PREC_MIXED && A->L.has_lowp;
-    const int var = o->variant == 1 ? 1 : 2;
+    const int tv = A->tuned_variant[mixed ? 1 : 0];
+    const int var = o->variant == 1 ? 1 : (o->variant == 2 ? 2 : (tv == 1 ? 1 : 2));
     if (A->L.tilem == 0) return MF_ERR_INVALID;
     const int G = choose_grid(A, o, ((long long)var << 8) | (mixed ? 1 : 0));
     if (G > A->L.tilem) return MF_ERR_INVALID;
@@ -457,4 +200,48 @@ extern "C" mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, 
     free_dev(db);
     free_dev(dx);
     return st;
+}
+
+extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
+                                    int *best_variant, double *best_ms_per_iter, void *stream)
+{
+    if (!A || !d_b || !d_x) return MF_ERR_INVALID;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const bool mixed = precision == MF_PREC_MIXED && A->L.has_lowp;
+    const int cands[] = {8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 512};
+    double best = 1e30;
+    int bg = 0, bv = 2;
+    for (int var = 1; var <= 2; var++) {
+        for (int G : cands) {
+            if (G > A->L.tilem || G > 2 * A->num_cus) continue;
+            mf_solve_options o;
+            mf_solve_options_default(&o);
+            o.precision = precision;
+            o.variant = var;
+            o.grid = G;
+            o.fixed_iterations = 1;
+            o.max_iter = 40;
+            mf_solve_info info;
+            double t = 1e30;
+            mf_status st = MF_SUCCESS;
+            for (int rep = 0; rep < 2 && st == MF_SUCCESS; rep++) {
+                st = cg_launch(A, d_b, d_x, &o, &info, s);
+                if (st == MF_SUCCESS) t = std::min(t, info.time_ms / std::max(1, info.iterations));
+            }
+            if (st != MF_SUCCESS) continue;  // e.g. rows per workgroup beyond the kernel's capacity
+            if (t < best) {
+                best = t;
+                bg = G;
+                bv = var;
+            }
+        }
+    }
+    if (bg == 0) return MF_ERR_TOO_LARGE;
+    for (int var = 1; var <= 2; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
+    A->tuned_variant[mixed ? 1 : 0] = bv;
+    if (best_grid) *best_grid = bg;
+    if (best_variant) *best_variant = bv;
+    if (best_ms_per_iter) *best_ms_per_iter = best;
+    return MF_SUCCESS;
 }

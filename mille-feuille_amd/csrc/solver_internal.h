@@ -29,6 +29,7 @@ struct DeviceLayout {
     float *v32 = nullptr;
     uint16_t *v16 = nullptr;
     int8_t *v8 = nullptr;
+    int *src = nullptr;             // per entry: position in the input CSR arrays
     bool has_lowp = false;
     bool need_p64_mixed = false;
     int64_t tiles_by_class[4] = {0, 0, 0, 0};
@@ -70,6 +71,8 @@ int spmv_grid(const mf_matrix *A);
 struct mf_matrix {
     int device = 0;
     int class_policy = 0;
+    int keep_mixed = 1;
+    std::vector<int> rowptr;  // host copy of the CSR row pointer (row scales, value updates)
     int num_cus = 0;
     int max_lds = 65536;
     mf::TileStruct ts;
@@ -77,4 +80,5 @@ struct mf_matrix {
     mf::DeviceLayout L;
     mf::Workspace ws;
     std::map<long long, int> tuned_grid;  // autotuner cache: key -> workgroups
+    int tuned_variant[2] = {0, 0};        // autotuner cache: fp64 / mixed CG variant (0 = default)
 };
