@@ -247,6 +247,14 @@ mf_status mf_matrix_update_values(mf_matrix *A, const double *val, void *stream)
 mf_status mf_pattern_outer(mf_matrix *A, const double *d_u, const double *d_v, double scale, double *d_out,
                            void *stream);
 
+/* Partial-convergence class of every 16-element segment of p, as computed inside the
+ * solver (main-cg-mixed.cu:1323-1371 restated deterministically): out[s] in
+ * {0 fp64, 1 fp32, 2 fp16, 3 int8, 4 skip}.  mode 2: the reference's |p_new - p_old|
+ * thresholds, highest precision in the segment wins; mode 1: max|p_new| / rms relative
+ * to 2^-13 and 2^-24. */
+mf_status mf_dyn_classify(const double *d_p_old, const double *d_p_new, int n, int mode, double rms,
+                          unsigned char *d_out, void *stream);
+
 /* y = A x (or A^T x) on device pointers.  precision MF_PREC_MIXED applies the tile
  * classes (binary32 products, FP64 sums); MF_PREC_FP64 reads the FP64 stream. */
 mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int precision, int transpose,

@@ -128,6 +128,7 @@ _SIGS = {
     "mf_matrix_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(MatrixStats)]),
     "mf_matrix_update_values": (ctypes.c_int, [vp, f64p, vp]),
     "mf_spmv": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+    "mf_dyn_classify": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp]),
     "mf_pattern_outer": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, vp, vp]),
     "mf_cg_solve": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo), vp]),
     "mf_bicgstab_solve": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo), vp]),

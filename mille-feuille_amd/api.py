@@ -331,6 +331,19 @@ class TiledMatrix:
         return self._solve("mf_bicgstab_solve_multikernel", b, x0, solve_options(**kw))
 
 
+def dyn_classify(p_old, p_new, mode=2, rms=1.0, device=0):
+    """Partial-convergence class per 16-element segment, computed on the GPU."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    po = torch.as_tensor(np.ascontiguousarray(p_old, np.float64), device=dev)
+    pn = torch.as_tensor(np.ascontiguousarray(p_new, np.float64), device=dev)
+    n = po.numel()
+    out = torch.zeros((n + 15) // 16, dtype=torch.uint8, device=dev)
+    check(L.lib().mf_dyn_classify(ctypes.c_void_p(po.data_ptr()), ctypes.c_void_p(pn.data_ptr()), n, int(mode),
+                                  float(rms), ctypes.c_void_p(out.data_ptr()), _stream_handle()), "mf_dyn_classify")
+    return out.cpu().numpy()
+
+
 def device_name(device=0):
     buf = ctypes.create_string_buffer(256)
     check(L.lib().mf_device_name(int(device), buf, 256), "mf_device_name")

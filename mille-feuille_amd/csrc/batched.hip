@@ -110,6 +110,9 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
         part.need64 = a.need64;
         part.need32 = a.need32;
         part.max_repl = a.max_repl;
+        part.dvis = nullptr;
+        part.dyn_mode = 0;
+        part.n_sys = a.sys_rows[s];
         CgResult res;
         if (SINGLE) res = cg_core<RPT, 2, MIXED, 1024, BlockOps, U>(part, R, bsy, scratch, q, xs);
         else res = cg_core<RPT, 2, MIXED, 1024, GridOps, U>(part, R, gsy, scratch, q, xs);

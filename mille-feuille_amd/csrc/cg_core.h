@@ -23,6 +23,8 @@ struct CgArgs {
     int need64;
     int need32;
     int max_repl;
+    uint8_t *dvis;
+    int dyn_mode;
 };
 
 enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
@@ -62,6 +64,26 @@ struct CgPart {
     float *p32;
     double tol2, delta2;
     int maxit, fixed, use_x0, need64, need32, max_repl;
+    uint8_t *dvis;   // dynamic class per column tile (written by the owner of the tile row)
+    int dyn_mode;    // 1 magnitude-relative, 2 the reference's |dp| thresholds
+    int n_sys;       // rows of the system (for the RMS of p)
+};
+
+// This workgroup's share of x: in LDS (XREG = false) or in registers (XREG = true).
+template <int RPT, bool XREG>
+struct XVec {
+    double reg[XREG ? RPT : 1];
+    double *lds;
+    __device__ __forceinline__ double get(int k, int loc) const
+    {
+        if constexpr (XREG) return reg[k];
+        else return lds[loc];
+    }
+    __device__ __forceinline__ void set(int k, int loc, double v)
+    {
+        if constexpr (XREG) reg[k] = v;
+        else lds[loc] = v;
+    }
 };
 
 struct CgResult {
@@ -82,16 +104,30 @@ __device__ __forceinline__ void publish_p(const CgPart &a, Rows<RPT, BLK> &R, co
     }
 }
 
+// Dynamic class of every 16-row segment this thread's lanes hold, written by the
+// segment's first lane.  All 16 lanes of a segment must call it together.
+template <int RPT, int BLK>
+__device__ __forceinline__ void classify_segments(const CgPart &a, Rows<RPT, BLK> &R, const double (&p_old)[RPT],
+                                                  const double (&p_new)[RPT], double rms)
+{
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const int c = dyn_segment_class(a.dyn_mode, p_old[k], p_new[k], rms);
+        if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = (uint8_t)c;
+    }
+}
+
 // r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old)
 // when fix_p; returns the r'r sum.  One SpMV and two syncs.
-template <int RPT, int BLK, class Sync>
+template <int RPT, int BLK, class Sync, bool XREG>
 __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *q, double *scratch,
-                                 const double *xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
+                                 const XVec<RPT, XREG> &xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
 {
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
     double dummy[1] = {0.0};
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
@@ -116,9 +152,13 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
 
 // The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
 // a.x; every member returns the same CgResult.
-template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = (RPT >= 8 ? 2 : 4)>
-__device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q, double *xs)
+template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = (RPT >= 8 ? 2 : 4), bool XREG = false,
+          bool DYN = false>
+__device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
+                            double *xs_lds)
 {
+    XVec<RPT, XREG> xs;
+    xs.lds = xs_lds;
     CgResult res;
     res.it = 0;
     res.repl = 0;
@@ -141,24 +181,31 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
         const double bk = a.b[row];
-        xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
+        xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
         r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
         p[k] = r[k];
         part[0] += r[k] * r[k];
         part[1] += bk * bk;
     }
     publish_p<RPT, BLK>(a, R, p);
-    double rr = 0.0, bb = 0.0, rbase = 0.0;
+    if (DYN) {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = 0;
+    }
+    double rr = 0.0, bb = 0.0, rbase = 0.0, pp = 0.0;
     bool alive = sy.template sum<2>(part, true, scratch);
     if (alive) {
         rr = part[0];
         bb = part[1];
         rbase = rr;
+        pp = rr;
         if (bb == 0.0) {
             R.refresh();
 #pragma unroll
             for (int k = 0; k < RPT; k++)
-                if (R.ok(k)) xs[R.loc(k)] = 0.0;
+                if (R.ok(k)) xs.set(k, R.loc(k), 0.0);
             rr = 0.0;
             res.status = ST_OK;
         }
@@ -170,7 +217,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         if (!a.fixed) {
             bool conv = rr <= a.tol2 * bb;
             if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                if (!replace_residual<RPT, BLK>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
+                if (!replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
                     res.status = ST_TIMEOUT;
                     break;
                 }
@@ -190,7 +237,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         // q = A p
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, a.c0, a.c1, R.row0, a.p64, a.p32, q);
+        spmv_chunks<MIXED, U, DYN>(a.A, a.c0, a.c1, R.row0, a.p64, a.p32, q);
         __syncthreads();
         R.refresh();
         if (VAR == 1) {
@@ -212,7 +259,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
                 if (!R.ok(k)) continue;
-                xs[R.loc(k)] += alpha * p[k];
+                xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
                 r[k] -= alpha * q[R.loc(k)];
                 e[0] += r[k] * r[k];
             }
@@ -222,14 +269,21 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             }
             const double beta = e[0] / rr;
             rr = e[0];
+            double pold[DYN ? RPT : 1];
+            double z[2] = {0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < RPT; k++) p[k] = r[k] + beta * p[k];
+            for (int k = 0; k < RPT; k++) {
+                if (DYN) pold[DYN ? k : 0] = p[k];
+                p[k] = r[k] + beta * p[k];
+                z[1] += p[k] * p[k];
+            }
+            if constexpr (DYN) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
             publish_p<RPT, BLK>(a, R, p);
-            double z[1] = {0.0};
-            if (!sy.template sum<1>(z, true, scratch)) {
+            if (!sy.template sum<2>(z, true, scratch)) {
                 res.status = ST_TIMEOUT;
                 break;
             }
+            pp = z[1];
         } else {
             double d[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -252,22 +306,27 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             double rr_new = rr - 2.0 * alpha * d[1] + alpha * alpha * d[2];
             if (!(rr_new > 0.0)) rr_new = 0.0;
             const double beta = rr_new / rr;
-            double e[1] = {0.0};
+            double e[2] = {0.0, 0.0};
+            double pold[DYN ? RPT : 1];
             R.refresh();
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
+                if (DYN) pold[DYN ? k : 0] = p[k];
                 if (!R.ok(k)) continue;
-                xs[R.loc(k)] += alpha * p[k];
+                xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
                 r[k] -= alpha * q[R.loc(k)];
                 p[k] = r[k] + beta * p[k];
                 e[0] += r[k] * r[k];
+                e[1] += p[k] * p[k];
             }
+            if constexpr (DYN) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
             publish_p<RPT, BLK>(a, R, p);
-            if (!sy.template sum<1>(e, true, scratch)) {
+            if (!sy.template sum<2>(e, true, scratch)) {
                 res.status = ST_TIMEOUT;
                 break;
             }
             rr = e[0];
+            pp = e[1];
         }
         res.it++;
     }
@@ -280,26 +339,26 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             pd[k] = p[k];
         }
         double rr_t = 0.0;
-        if (replace_residual<RPT, BLK>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false)) res.rt = rr_t;
+        if (replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false)) res.rt = rr_t;
         else res.status = ST_TIMEOUT;
     } else {
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
     }
     res.rr = rr;
     res.bb = bb;
     return res;
 }
 
-template <int RPT, int VAR, bool MIXED, int BLK>
+template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN>
 __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
 {
     extern __shared__ double smem[];
     double *scratch = smem;                  // 64 doubles for reductions
     double *q = smem + 64;                   // rows_cap doubles: SpMV result
-    double *xs = smem + 64 + a.A.rows_cap;   // rows_cap doubles: this workgroup's x
+    double *xs = smem + 64 + a.A.rows_cap;   // rows_cap doubles: this workgroup's x (XREG = false)
     const int G = a.gs.G;
     const int wg = logical_wg(blockIdx.x, G);
     const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
@@ -327,7 +386,10 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     sy.gs = a.gs;
     sy.wg = wg;
     sy.epoch = 0;
-    CgResult res = cg_core<RPT, VAR, MIXED, BLK>(part, R, sy, scratch, q, xs);
+    part.dvis = a.dvis;
+    part.dyn_mode = a.dyn_mode;
+    part.n_sys = a.A.n;
+    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, (RPT >= 8 ? 2 : 4), XREG, DYN>(part, R, sy, scratch, q, xs);
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)res.it;
         a.info[1] = res.rr;

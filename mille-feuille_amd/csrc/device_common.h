@@ -22,6 +22,7 @@ struct DevMatrix {
     const float *v32;
     const _Float16 *v16;
     const int8_t *v8;
+    const uint8_t *dvis;        // per column tile: dynamic class (0 fp64 .. 3 int8, 4 skip); may be null
     const int *wg_tr;           // G+1 tile-row cut points of the persistent partition
     const long long *wg_chunk;  // G+1 chunk cut points
     int n;
@@ -156,8 +157,10 @@ __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long
 //   MIXED = false: binary64 values times x64.
 //   MIXED = true : class-0 chunks as above; class 1/2/3 chunks read their value at that
 //                  precision, multiply by x32 in binary32 and add the product in binary64.
+//   DYN          : the chunk's class is lowered to its column tile's dynamic class
+//                  (A.dvis) when that is lower; dynamic class 4 skips the chunk.
 //   kUnroll      : chunks per 16-lane slot in flight (loads issued before first use).
-template <bool MIXED, int kUnroll = 4>
+template <bool MIXED, int kUnroll = 4, bool DYN = false>
 __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, long long c0, long long c1, int row0,
                                             const double *__restrict__ x64, const float *__restrict__ x32,
                                             double *q)
@@ -172,10 +175,16 @@ __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, long long c0, lo
             long long c = cb + 4 * u + slot;
             m[u] = c < c1 ? A.chunks[c] : make_int4(0, 0, 0, -1);
         }
+        int cls[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            cls[u] = (m[u].w >> 4) & 3;
+            if (DYN && m[u].w >= 0) cls[u] = max(cls[u], (int)A.dvis[m[u].y >> 4]);
+        }
         int id[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-            bool act = m[u].w >= 0 && k <= (m[u].w & 15);
+            bool act = m[u].w >= 0 && k <= (m[u].w & 15) && (!DYN || cls[u] < 4);
             id[u] = act ? (int)A.eidx[(long long)m[u].x + k] : -1;
         }
         double prod[kUnroll];
@@ -185,11 +194,10 @@ __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, long long c0, lo
             if (id[u] >= 0) {
                 const long long e = (long long)m[u].x + k;
                 const int col = m[u].y + (id[u] & 15);
-                const int cls = (m[u].w >> 4) & 3;
-                if (!MIXED || cls == 0) {
+                if (!MIXED || cls[u] == 0) {
                     prod[u] = A.v64[e] * x64[col];
                 } else {
-                    prod[u] = (double)(val_lowp(A, cls, e) * x32[col]);
+                    prod[u] = (double)(val_lowp(A, cls[u], e) * x32[col]);
                 }
             }
         }
@@ -197,6 +205,41 @@ __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, long long c0, lo
         for (int u = 0; u < kUnroll; u++)
             if (id[u] >= 0) atomicAdd(&q[m[u].z - row0 + (id[u] >> 4)], prod[u]);
     }
+}
+
+// Dynamic ("partial convergence") class of one 16-row segment from the element values
+// of that segment held by 16 consecutive lanes (width-16 shuffles; every lane returns
+// the segment's class, 0 fp64 .. 3 int8, 4 skip).
+//   mode 2: the reference's thresholds on d = |p_new - p_old| (main-cg-mixed.cu:1334-1351)
+//           -- d > 1e-5 fp64, (1e-6,1e-5] fp32, (1e-7,1e-6] fp16, (1e-8,1e-7] int8, else
+//           skip -- with the segment taking the highest precision any element asks for.
+//   mode 1: magnitude relative to the RMS of p: m = max|p_new| / rms; m > 2^-13 keeps the
+//           tile's class, 2^-24 < m <= 2^-13 allows binary16, m <= 2^-24 skips the tile
+//           (its contribution is below binary32 rounding of the total).
+__device__ __forceinline__ int dyn_code_ref(double d)
+{
+    if (d <= 1e-7 && d > 1e-8) return 3;
+    if (d <= 1e-6 && d > 1e-7) return 2;
+    if (d <= 1e-5 && d > 1e-6) return 1;
+    if (d > 1e-5) return 0;
+    return 4;
+}
+
+__device__ __forceinline__ int dyn_segment_class(int mode, double p_old, double p_new, double rms)
+{
+    int code;
+    if (mode == 2) {
+        code = dyn_code_ref(fabs(p_new - p_old));
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) code = min(code, __shfl_xor(code, o, 16));
+    } else {
+        double mx = fabs(p_new);
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 16));
+        const double m = rms > 0.0 ? mx / rms : 0.0;
+        code = m > 0x1p-13 ? 0 : (m > 0x1p-24 ? 2 : 4);
+    }
+    return code;
 }
 
 // Opaque copy of a value: stops the compiler from hoisting per-row addresses of the

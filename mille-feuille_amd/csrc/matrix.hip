@@ -102,7 +102,7 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
         L.nnz_by_class[cls[t]] += ts.tile_off[t + 1] - ts.tile_off[t];
         if (cls[t] != MF_FP64) any_lowp = true;
     }
-    if (keep_mixed && any_lowp) {
+    if (keep_mixed) {
         v32.assign(ts.nnz, 0.f);
         v16.assign(ts.nnz, 0);
         v8.assign(ts.nnz, 0);
@@ -121,17 +121,24 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
                 double v = val[ts.src[e]];
                 v64[e] = v;
                 if (!v32.empty()) {
-                    switch (cls[t]) {
-                    case MF_FP32: v32[e] = (float)v; break;
-                    case MF_FP16: v16[e] = f64_to_f16_rne(v); break;
-                    case MF_INT8: v8[e] = (int8_t)v; break;
-                    default: break;
+                    // the tile's own class holds its rounding of the value; the lower
+                    // precisions hold the copies of main-cg-mixed.cu:2333-2350 (f32 RNE,
+                    // f16 of the f32, i8 truncated), read only under dynamic classes
+                    const float f = (float)v;
+                    v32[e] = f;
+                    v16[e] = cls[t] == MF_FP16 ? f64_to_f16_rne(v) : f32_to_f16_rne(f);
+                    if (cls[t] == MF_INT8) {
+                        v8[e] = (int8_t)v;
+                    } else {
+                        const double tv = std::trunc(v);
+                        v8[e] = std::isnan(v) ? 0 : (int8_t)(tv > 127.0 ? 127 : (tv < -128.0 ? -128 : tv));
                     }
                 }
             }
         }
     }
-    L.has_lowp = !v32.empty();
+    L.has_lowp = any_lowp && !v32.empty();
+    L.has_copies = !v32.empty();
     L.need_p64_mixed = L.tiles_by_class[MF_FP64] > 0;
     mf_status st;
     if ((st = upload((void **)&L.chunks, ch.data(), sizeof(int4) * C, s)) != MF_SUCCESS) return st;
@@ -142,7 +149,7 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
         for (int64_t e = 0; e < ts.nnz; e++) src32[e] = (int)ts.src[e];
         if ((st = upload((void **)&L.src, src32.data(), sizeof(int) * ts.nnz, s)) != MF_SUCCESS) return st;
     }
-    if (L.has_lowp) {
+    if (L.has_copies) {
         if ((st = upload((void **)&L.v32, v32.data(), sizeof(float) * ts.nnz, s)) != MF_SUCCESS) return st;
         if ((st = upload((void **)&L.v16, v16.data(), sizeof(uint16_t) * ts.nnz, s)) != MF_SUCCESS) return st;
         if ((st = upload((void **)&L.v8, v8.data(), ts.nnz, s)) != MF_SUCCESS) return st;
@@ -153,7 +160,7 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     L.alg_bytes_f64 = 12.0 * (double)T + 9.0 * (double)ts.nnz;
     L.alg_bytes_mixed = 12.0 * (double)T;
     for (int c = 0; c < 4; c++) L.alg_bytes_mixed += (sv[c] + 1.0) * (double)L.nnz_by_class[c];
-    L.device_bytes = (double)C * 16 + (double)ts.nnz * (1 + 8 + (L.has_lowp ? 7 : 0));
+    L.device_bytes = (double)C * 16 + (double)ts.nnz * (1 + 8 + 4 + (L.has_copies ? 7 : 0));
     return MF_SUCCESS;
 }
 
@@ -207,6 +214,7 @@ mfk::DevMatrix DeviceLayout::dev(const Partition &P) const
     d.v32 = v32;
     d.v16 = (const _Float16 *)v16;
     d.v8 = v8;
+    d.dvis = nullptr;
     d.wg_tr = P.wg_tr;
     d.wg_chunk = P.wg_chunk;
     d.n = n_rows;
@@ -241,6 +249,12 @@ mf_status Workspace::ensure(int64_t n, int G, hipStream_t s)
             return MF_ERR_ALLOC;
         cap_g = G;
     }
+    const int64_t nseg = n / 16 + 2;
+    if (nseg > cap_dvis) {
+        free_dev(dvis);
+        if (hipMalloc((void **)&dvis, (size_t)nseg) != hipSuccess) return MF_ERR_ALLOC;
+        cap_dvis = nseg;
+    }
     if (!ctl) {
         if (hipMalloc((void **)&ctl, 256) != hipSuccess) return MF_ERR_ALLOC;
         if (hipMalloc((void **)&info, sizeof(double) * 32) != hipSuccess) return MF_ERR_ALLOC;
@@ -260,6 +274,7 @@ Workspace::~Workspace()
     free_dev(recs);
     free_dev(ctl);
     free_dev(info);
+    free_dev(dvis);
 }
 
 }  // namespace mf
@@ -297,6 +312,19 @@ __global__ void pattern_outer_kernel(DevMatrix A, const int *__restrict__ src, l
     const long long e = (long long)m.x + k;
     const int id = A.eidx[e];
     out[src[e] - base] = scale * u[m.z + (id >> 4)] * v[m.y + (id & 15)];
+}
+
+// Dynamic class of every 16-element segment (the classification used inside the
+// solver), one thread per element.
+__global__ void dyn_classify_kernel(const double *__restrict__ p_old, const double *__restrict__ p_new, int n,
+                                    int mode, double rms, uint8_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nseg = (n + 15) / 16;
+    if (i / 16 >= nseg) return;  // whole segments only, so the 16-lane shuffle is complete
+    const double po = i < n ? p_old[i] : 0.0, pn = i < n ? p_new[i] : 0.0;
+    const int c = dyn_segment_class(mode, po, pn, rms);
+    if ((i & 15) == 0) out[i >> 4] = (uint8_t)c;
 }
 
 __global__ void to_f32_kernel(const double *__restrict__ x, float *__restrict__ y, int n)
@@ -442,6 +470,17 @@ extern "C" mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int p
                                   (int)lds);
         hipLaunchKernelGGL(mfk::spmv_kernel<false>, dim3(G), dim3(1024), lds, s, dm, d_x, A->ws.p32, d_y);
     }
+    return hipGetLastError() == hipSuccess ? MF_SUCCESS : MF_ERR_HIP;
+}
+
+extern "C" mf_status mf_dyn_classify(const double *d_p_old, const double *d_p_new, int n, int mode, double rms,
+                                     unsigned char *d_out, void *stream)
+{
+    if (!d_p_old || !d_p_new || !d_out || n < 0 || (mode != 1 && mode != 2)) return MF_ERR_INVALID;
+    if (n == 0) return MF_SUCCESS;
+    const int threads = (n + 15) / 16 * 16;
+    hipLaunchKernelGGL(mfk::dyn_classify_kernel, dim3((threads + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       d_p_old, d_p_new, n, mode, rms, d_out);
     return hipGetLastError() == hipSuccess ? MF_SUCCESS : MF_ERR_HIP;
 }
 

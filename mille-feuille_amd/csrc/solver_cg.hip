@@ -54,28 +54,33 @@ using CgKernel = void (*)(mfk::CgArgs);
 
 // Rows per thread and block size: 1024 threads, up to 8 rows per thread (r and p in
 // at most 128 VGPRs per wave, 16 waves per CU).
-template <int VAR, bool MIXED>
-static CgKernel pick_cg(int rpt, int blk)
+template <int VAR, bool MIXED, bool XREG, bool DYN>
+static CgKernel pick_cg(int rpt)
 {
-    if (blk == 1024) {
-        switch (rpt) {
-        case 1: return mfk::cg_kernel<1, VAR, MIXED, 1024>;
-        case 2: return mfk::cg_kernel<2, VAR, MIXED, 1024>;
-        case 4: return mfk::cg_kernel<4, VAR, MIXED, 1024>;
-        case 8: return mfk::cg_kernel<8, VAR, MIXED, 1024>;
-        default: return nullptr;
-        }
+    switch (rpt) {
+    case 1: return mfk::cg_kernel<1, VAR, MIXED, 1024, XREG, DYN>;
+    case 2: return mfk::cg_kernel<2, VAR, MIXED, 1024, XREG, DYN>;
+    case 4: return mfk::cg_kernel<4, VAR, MIXED, 1024, XREG, DYN>;
+    case 8: return mfk::cg_kernel<8, VAR, MIXED, 1024, XREG, DYN>;
+    default: return nullptr;
     }
-    return nullptr;
 }
 
-static CgKernel cg_kernel_for(int rpt, int blk, int var, bool mixed)
+// Dynamic classes are instantiated for the mixed, two-sync variant only.
+static CgKernel cg_kernel_for(int rpt, int var, bool mixed, bool xreg, bool dyn)
 {
-    if (var == 1) return mixed ? pick_cg<1, true>(rpt, blk) : pick_cg<1, false>(rpt, blk);
-    return mixed ? pick_cg<2, true>(rpt, blk) : pick_cg<2, false>(rpt, blk);
+    if (dyn) {
+        if (!mixed || var != 2) return nullptr;
+        return xreg ? pick_cg<2, true, true, true>(rpt) : pick_cg<2, true, false, true>(rpt);
+    }
+    if (xreg) {
+        if (var == 1) return mixed ? pick_cg<1, true, true, false>(rpt) : pick_cg<1, false, true, false>(rpt);
+        return mixed ? pick_cg<2, true, true, false>(rpt) : pick_cg<2, false, true, false>(rpt);
+    }
+    if (var == 1) return mixed ? pick_cg<1, true, false, false>(rpt) : pick_cg<1, false, false, false>(rpt);
+    return mixed ? pick_cg<2, true, false, false>(rpt) : pick_cg<2, false, false, false>(rpt);
 }
 
-// Smallest supported (rows per thread, block) covering rows_cap rows.
 }  // namespace mf
 
 using namespace mf;
@@ -106,9 +111,10 @@ int default_grid(const mf_matrix *A, int maxG)
 mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *o, mf_solve_info *info,
                     hipStream_t s)
 {
-    const bool mixed = o->precision == MF_PREC_MIXED && A->L.has_lowp;
+    const bool dyn = o->precision == MF_PREC_MIXED && o->dynamic_precision != 0 && A->L.has_copies;
+    const bool mixed = o->precision == MF_PREC_MIXED && (A->L.has_lowp || dyn);
     const int tv = A->tuned_variant[mixed ? 1 : 0];
-    const int var = o->variant == 1 ? 1 : (o->variant == 2 ? 2 : (tv == 1 ? 1 : 2));
+    const int var = dyn ? 2 : (o->variant == 1 ? 1 : (o->variant == 2 ? 2 : (tv == 1 ? 1 : 2)));
     if (A->L.tilem == 0) return MF_ERR_INVALID;
     const int G = choose_grid(A, o, ((long long)var << 8) | (mixed ? 1 : 0));
     if (G > A->L.tilem) return MF_ERR_INVALID;
@@ -117,9 +123,15 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     if (st != MF_SUCCESS) return st;
     int rpt = 0, B = 0;
     if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
-    const size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    // x in LDS next to q when both fit the per-workgroup LDS, else x in registers
+    size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    bool xreg = false;
+    if (lds > (size_t)A->max_lds) {
+        xreg = true;
+        lds = (64 + (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    }
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    CgKernel k = cg_kernel_for(rpt, B, var, mixed);
+    CgKernel k = cg_kernel_for(rpt, var, mixed, xreg, dyn);
     if (!k) return MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
 
@@ -143,6 +155,10 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
+    a.dvis = A->ws.dvis;
+    a.dyn_mode = dyn ? (o->dynamic_precision == 2 ? 2 : 1) : 0;
+    a.A.dvis = dyn ? A->ws.dvis : nullptr;
+    if (dyn) a.need64 = 1;  // dynamic classes may keep any tile at binary64
     float ms = 0;
     double h[32];
     if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;

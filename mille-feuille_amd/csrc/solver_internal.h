@@ -31,6 +31,7 @@ struct DeviceLayout {
     int8_t *v8 = nullptr;
     int *src = nullptr;             // per entry: position in the input CSR arrays
     bool has_lowp = false;
+    bool has_copies = false;        // v32/v16/v8 hold every entry (lossy where the class is higher)
     bool need_p64_mixed = false;
     int64_t tiles_by_class[4] = {0, 0, 0, 0};
     int64_t nnz_by_class[4] = {0, 0, 0, 0};
@@ -54,12 +55,14 @@ struct Workspace {
     float *p32 = nullptr, *v32 = nullptr, *s32 = nullptr;
     unsigned long long *recs = nullptr;
     unsigned *ctl = nullptr;
+    uint8_t *dvis = nullptr;   // dynamic class per column tile
     double *info = nullptr;
     Workspace() = default;
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
     ~Workspace();
     mf_status ensure(int64_t n, int G, hipStream_t s);
+    int64_t cap_dvis = 0;
 };
 
 mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s);

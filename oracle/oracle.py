@@ -392,3 +392,37 @@ def read_records(path):
         out[name] = np.frombuffer(raw, t, cnt, pos).copy()
         pos += cnt * t.itemsize
     return out
+
+
+# ---------------------------------------------------------------------------------
+# Dynamic (partial-convergence) classes in the build's code numbering
+#   0 fp64, 1 fp32, 2 fp16, 3 int8, 4 skip
+# ---------------------------------------------------------------------------------
+REF_DVIS_TO_CODE = {0: 0, 4: 1, 3: 2, 2: 3, 1: 4}
+
+
+def dyn_classes_ref(p_old, p_new):
+    """Reference thresholds (main-cg-mixed.cu:1334-1369), per 16-element segment,
+    deterministic aggregate (oracle_dyn_class), mapped to build codes."""
+    n = len(p_old)
+    out = []
+    for s0 in range(0, n, 16):
+        po = np.zeros(16)
+        pn = np.zeros(16)
+        m = min(16, n - s0)
+        po[:m] = p_old[s0:s0 + m]
+        pn[:m] = p_new[s0:s0 + m]
+        out.append(REF_DVIS_TO_CODE[dyn_class(po, pn)])
+    return np.array(out, np.uint8)
+
+
+def dyn_classes_rel(p_new, rms):
+    """Magnitude rule: m = max|p_new| / rms per segment; m > 2^-13 -> 0, m > 2^-24 -> 2,
+    else 4 (skip)."""
+    n = len(p_new)
+    out = []
+    for s0 in range(0, n, 16):
+        mx = float(np.max(np.abs(p_new[s0:s0 + 16]))) if s0 < n else 0.0
+        m = mx / rms if rms > 0 else 0.0
+        out.append(0 if m > 2.0 ** -13 else (2 if m > 2.0 ** -24 else 4))
+    return np.array(out, np.uint8)
