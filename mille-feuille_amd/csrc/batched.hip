@@ -29,7 +29,7 @@ namespace mfk {
 
 struct BatchArgs {
     DevMatrix A;
-    const long long *tr_chunk;  // tilem+1
+    const int *sys_rng0;        // count: first range (of A.wg_chunk) of each system's members
     const int *sys_tr;          // count+1 tile-row starts
     const int *sys_rows;        // count
     const int *order;           // systems of every pool, largest first
@@ -96,8 +96,7 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
         R.refresh();
         CgPart part;
         part.A = a.A;
-        part.c0 = a.tr_chunk[mt0];
-        part.c1 = a.tr_chunk[mt1];
+        part.cr = a.A.wg_chunk + 4 * (size_t)(a.sys_rng0[s] + member);
         part.b = a.b;
         part.x = a.x;
         part.p64 = a.p64;
@@ -158,11 +157,11 @@ struct mf_batch {
     std::vector<int64_t> row_off;      // count+1 (padded offsets)
     std::vector<int64_t> nnz;
     int rows_per_wg = 8192;            // tuned bound on a member's rows
-    int unroll = 2;                    // chunks in flight per 16-lane slot
     // device tables
     int *d_sys_tr = nullptr, *d_sys_rows = nullptr, *d_order = nullptr;
     int *d_pool_W = nullptr, *d_pool_wg0 = nullptr, *d_pool_sys0 = nullptr;
-    long long *d_tr_chunk = nullptr;
+    int *d_sys_rng0 = nullptr;         // per system: first member range of `part`
+    mf::Partition part;                // member ranges of the plan, class-sorted chunks
     unsigned *d_queue = nullptr;
     double *d_info = nullptr;
     int npools = 0, G = 0;
@@ -177,9 +176,10 @@ struct mf_batch {
 
 void mf_batch::free_plan()
 {
-    for (void *p : {(void *)d_order, (void *)d_pool_W, (void *)d_pool_wg0, (void *)d_pool_sys0})
+    for (void *p : {(void *)d_order, (void *)d_pool_W, (void *)d_pool_wg0, (void *)d_pool_sys0, (void *)d_sys_rng0})
         mf::free_dev(p);
-    d_order = d_pool_W = d_pool_wg0 = d_pool_sys0 = nullptr;
+    d_order = d_pool_W = d_pool_wg0 = d_pool_sys0 = d_sys_rng0 = nullptr;
+    mf::free_partition(part);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
@@ -191,7 +191,7 @@ void mf_batch::free_plan()
 mf_batch::~mf_batch()
 {
     free_plan();
-    for (void *p : {(void *)d_sys_tr, (void *)d_sys_rows, (void *)d_tr_chunk, (void *)d_queue, (void *)d_info})
+    for (void *p : {(void *)d_sys_tr, (void *)d_sys_rows, (void *)d_queue, (void *)d_info})
         mf::free_dev(p);
     if (M) mf_matrix_destroy(M);
 }
@@ -253,9 +253,7 @@ extern "C" mf_status mf_batch_create(int count, const int *n_rows, const int *co
     for (int s = 0; s <= count; s++) tr[s] = (int)(B->row_off[s] / 16);
     hipStream_t hs = (hipStream_t)stream;
     if ((st = upload((void **)&B->d_sys_tr, tr.data(), sizeof(int) * (count + 1), hs)) != MF_SUCCESS ||
-        (st = upload((void **)&B->d_sys_rows, B->rows.data(), sizeof(int) * count, hs)) != MF_SUCCESS ||
-        (st = upload((void **)&B->d_tr_chunk, B->M->L.tr_chunk.data(), sizeof(long long) * (B->M->L.tilem + 1), hs)) !=
-            MF_SUCCESS) {
+        (st = upload((void **)&B->d_sys_rows, B->rows.data(), sizeof(int) * count, hs)) != MF_SUCCESS) {
         delete B;
         return st;
     }
@@ -305,6 +303,7 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
     if (B->planned) return MF_SUCCESS;
     const int R = B->rows_per_wg;
     std::vector<std::vector<int>> cls(4);
+    std::vector<int> sysW(B->count);
     for (int sidx = 0; sidx < B->count; sidx++) {
         const int tiles_rows = (int)((B->row_off[sidx + 1] - B->row_off[sidx]) / 16);
         int W = 1, ci = 0;
@@ -314,7 +313,21 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
         }
         if ((long long)(tiles_rows + W - 1) / W * 16 > R) return MF_ERR_TOO_LARGE;
         cls[ci].push_back(sidx);
+        sysW[sidx] = W;
     }
+    // member ranges (tile-row splits of every system, in matrix order) with their
+    // class-sorted chunks: member m of system s owns range rng0[s] + m
+    std::vector<int> cuts(1, 0), rng0(B->count);
+    for (int sidx = 0; sidx < B->count; sidx++) {
+        const int t0 = B->tr0[sidx];
+        const int t1 = (int)(B->row_off[sidx + 1] / 16);
+        const int W = sysW[sidx];
+        rng0[sidx] = (int)cuts.size() - 1;
+        for (int m = 1; m <= W; m++) cuts.push_back(t0 + (int)((long long)(t1 - t0) * m / W));
+    }
+    mf_status st0 = B->M->L.build_ranges(cuts, s, B->part);
+    if (st0 != MF_SUCCESS) return st0;
+    if ((st0 = upload((void **)&B->d_sys_rng0, rng0.data(), sizeof(int) * B->count, s)) != MF_SUCCESS) return st0;
     std::vector<int> pW, pwg0, psys0, order;
     std::vector<double> work;
     double total = 0;
@@ -366,14 +379,14 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
 
 using BatchKernel = void (*)(mfk::BatchArgs);
 
-template <bool MIXED, int U>
+template <bool MIXED>
 static BatchKernel pick_batch(int rpt)
 {
     switch (rpt) {
-    case 1: return mfk::batched_cg_kernel<1, MIXED, U>;
-    case 2: return mfk::batched_cg_kernel<2, MIXED, U>;
-    case 4: return mfk::batched_cg_kernel<4, MIXED, U>;
-    case 8: return mfk::batched_cg_kernel<8, MIXED, U>;
+    case 1: return mfk::batched_cg_kernel<1, MIXED, mfk::spmv_unroll<1>()>;
+    case 2: return mfk::batched_cg_kernel<2, MIXED, mfk::spmv_unroll<2>()>;
+    case 4: return mfk::batched_cg_kernel<4, MIXED, mfk::spmv_unroll<4>()>;
+    case 8: return mfk::batched_cg_kernel<8, MIXED, mfk::spmv_unroll<8>()>;
     default: return nullptr;
     }
 }
@@ -392,8 +405,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     if (opt) o = *opt;
     const bool mixed = o.precision == MF_PREC_MIXED && A->L.has_lowp;
     const int rpt = B->rows_per_wg / 1024;
-    BatchKernel k = B->unroll == 4 ? (mixed ? pick_batch<true, 4>(rpt) : pick_batch<false, 4>(rpt))
-                                   : (mixed ? pick_batch<true, 2>(rpt) : pick_batch<false, 2>(rpt));
+    BatchKernel k = mixed ? pick_batch<true>(rpt) : pick_batch<false>(rpt);
     if (!k) return MF_ERR_INVALID;
     const size_t lds = (64 + 2 * (size_t)B->rows_per_wg) * sizeof(double);
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
@@ -407,10 +419,9 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     if (B->G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
     if ((st = A->ws.ensure(A->L.n_rows, B->G, s)) != MF_SUCCESS) return st;
     mfk::BatchArgs a;
-    Partition P0;  // the batched kernel uses the tables below, not a persistent partition
-    P0.rows_cap = B->rows_per_wg;
-    a.A = A->L.dev(P0);
-    a.tr_chunk = B->d_tr_chunk;
+    a.A = A->L.dev(B->part);
+    a.A.rows_cap = B->rows_per_wg;
+    a.sys_rng0 = B->d_sys_rng0;
     a.sys_tr = B->d_sys_tr;
     a.sys_rows = B->d_sys_rows;
     a.order = B->d_order;

@@ -57,7 +57,7 @@ struct BlockOps {
 // vectors.
 struct CgPart {
     DevMatrix A;
-    long long c0, c1;
+    const long long *cr;  // class bounds of this workgroup's chunks (5 entries)
     const double *b;
     double *x;
     double *p64;
@@ -132,7 +132,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
     __syncthreads();
-    spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, a.c0, a.c1, R.row0, a.x, nullptr, q);
+    spmv_chunks<false, spmv_unroll<RPT>()>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
     __syncthreads();
     double part[1] = {0.0};
     R.refresh();
@@ -152,7 +152,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
 
 // The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
 // a.x; every member returns the same CgResult.
-template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = (RPT >= 8 ? 2 : 4), bool XREG = false,
+template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = spmv_unroll<RPT>(), bool XREG = false,
           bool DYN = false>
 __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
                             double *xs_lds)
@@ -170,7 +170,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     if (a.use_x0) {
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<false, U>(a.A, a.c0, a.c1, R.row0, a.x, nullptr, q);
+        spmv_chunks<false, U>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
         __syncthreads();
     }
     double part[2] = {0.0, 0.0};
@@ -195,12 +195,16 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = 0;
     }
     double rr = 0.0, bb = 0.0, rbase = 0.0, pp = 0.0;
+    bool dyn_on = DYN;
+    int dyn_it = 0;       // iteration and true r'r of the last dynamic-class check
+    double dyn_rr = 0.0;
     bool alive = sy.template sum<2>(part, true, scratch);
     if (alive) {
         rr = part[0];
         bb = part[1];
         rbase = rr;
         pp = rr;
+        dyn_rr = rr;
         if (bb == 0.0) {
             R.refresh();
 #pragma unroll
@@ -216,7 +220,12 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     while (alive && res.status == ST_NOTCONV) {
         if (!a.fixed) {
             bool conv = rr <= a.tol2 * bb;
-            if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
+            // under dynamic classes the true residual is also checked every 64
+            // iterations: skipped tiles can stall the recurrence without ever
+            // triggering the ratio test
+            const bool dyn_check = DYN && dyn_on && res.it - dyn_it >= 64;
+            if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase || dyn_check)) {
+                const double rr_rec = rr;
                 if (!replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
                     res.status = ST_TIMEOUT;
                     break;
@@ -224,6 +233,28 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                 res.repl++;
                 rbase = rr;
                 conv = rr <= a.tol2 * bb;
+                if constexpr (DYN) {
+                    // Lowered or skipped tiles have pulled the recurrence away from
+                    // b - A x, or the true residual has not fallen 4x since the last
+                    // check: return every tile to its static class for the rest of the
+                    // solve.  rr, rr_rec and dyn_rr are grid sums, so every workgroup
+                    // decides alike.
+                    const bool stalled = dyn_check && rr > 0.25 * dyn_rr;
+                    dyn_it = res.it;
+                    dyn_rr = rr;
+                    if (dyn_on && !conv && (rr > 4.0 * rr_rec || stalled)) {
+                        dyn_on = false;
+                        R.refresh();
+#pragma unroll
+                        for (int k = 0; k < RPT; k++)
+                            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = 0;
+                        double z[1] = {0.0};
+                        if (!sy.template sum<1>(z, true, scratch)) {
+                            res.status = ST_TIMEOUT;
+                            break;
+                        }
+                    }
+                }
             }
             if (conv) {
                 res.status = ST_OK;
@@ -237,7 +268,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         // q = A p
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<MIXED, U, DYN>(a.A, a.c0, a.c1, R.row0, a.p64, a.p32, q);
+        spmv_chunks<MIXED, U, DYN>(a.A, a.cr, R.row0, a.p64, a.p32, q, scratch);
         __syncthreads();
         R.refresh();
         if (VAR == 1) {
@@ -277,7 +308,9 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                 p[k] = r[k] + beta * p[k];
                 z[1] += p[k] * p[k];
             }
-            if constexpr (DYN) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
+            if constexpr (DYN) {
+                if (dyn_on) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
+            }
             publish_p<RPT, BLK>(a, R, p);
             if (!sy.template sum<2>(z, true, scratch)) {
                 res.status = ST_TIMEOUT;
@@ -319,7 +352,9 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                 e[0] += r[k] * r[k];
                 e[1] += p[k] * p[k];
             }
-            if constexpr (DYN) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
+            if constexpr (DYN) {
+                if (dyn_on) classify_segments<RPT, BLK>(a, R, pold, p, sqrt(pp / (double)a.n_sys));
+            }
             publish_p<RPT, BLK>(a, R, p);
             if (!sy.template sum<2>(e, true, scratch)) {
                 res.status = ST_TIMEOUT;
@@ -368,8 +403,7 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     R.refresh();
     CgPart part;
     part.A = a.A;
-    part.c0 = a.A.wg_chunk[wg];
-    part.c1 = a.A.wg_chunk[wg + 1];
+    part.cr = a.A.wg_chunk + 4 * (size_t)wg;
     part.b = a.b;
     part.x = a.x;
     part.p64 = a.p64;
@@ -389,7 +423,7 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.dvis = a.dvis;
     part.dyn_mode = a.dyn_mode;
     part.n_sys = a.A.n;
-    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, (RPT >= 8 ? 2 : 4), XREG, DYN>(part, R, sy, scratch, q, xs);
+    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT>(), XREG, DYN>(part, R, sy, scratch, q, xs);
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)res.it;
         a.info[1] = res.rr;

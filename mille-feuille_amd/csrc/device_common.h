@@ -10,13 +10,14 @@
 namespace mfk {
 
 constexpr int kTile = 16;
+constexpr int kMaxRowsPerWg = 8192;  // 1024 threads x 8 rows held in registers
 constexpr int kMaxRed = 4;     // doubles per grid reduction
 
-// One 16-entry chunk of a tile: entries [e0, e0+cnt) of the value stream, all in
-// tile row (rowbase/16) and column tile (colbase/16).  w = (cnt-1) | class << 4.
-// x = e0, y = colbase, z = rowbase, w = meta.
+// One 16-entry chunk of a tile: entries [e0, e0 + cnt) of the per-entry arrays, all in
+// one tile.  Header (12 bytes): x = e0, y = column tile | class << 28,
+// z = tile row | (cnt - 1) << 27; z < 0 marks "no chunk".
 struct DevMatrix {
-    const int4 *chunks;
+    const int3 *chunks;
     const uint8_t *eidx;        // (local row << 4) | local column, per entry
     const double *v64;
     const float *v32;
@@ -28,6 +29,13 @@ struct DevMatrix {
     int n;
     int rows_cap;               // max rows of one workgroup
 };
+
+constexpr unsigned kTileIdxMask = 0x07FFFFFFu;  // tile row / column index (n < 2^31)
+__device__ __forceinline__ int chunk_coltile(int3 h) { return (int)((unsigned)h.y & kTileIdxMask); }
+__device__ __forceinline__ int chunk_colbase(int3 h) { return chunk_coltile(h) << 4; }
+__device__ __forceinline__ int chunk_class(int3 h) { return ((unsigned)h.y >> 28) & 3; }
+__device__ __forceinline__ int chunk_rowbase(int3 h) { return (int)(((unsigned)h.z & kTileIdxMask) << 4); }
+__device__ __forceinline__ int chunk_last(int3 h) { return ((unsigned)h.z >> 27) & 15; }
 
 // Flagged-record grid barrier state.
 struct GridSync {
@@ -153,57 +161,148 @@ __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long
     return (float)A.v8[e];
 }
 
-// q[row - row0] += (A x)[row] for the chunks [c0, c1) of this workgroup.
-//   MIXED = false: binary64 values times x64.
-//   MIXED = true : class-0 chunks as above; class 1/2/3 chunks read their value at that
-//                  precision, multiply by x32 in binary32 and add the product in binary64.
-//   DYN          : the chunk's class is lowered to its column tile's dynamic class
-//                  (A.dvis) when that is lower; dynamic class 4 skips the chunk.
-//   kUnroll      : chunks per 16-lane slot in flight (loads issued before first use).
-template <bool MIXED, int kUnroll = 4, bool DYN = false>
-__device__ __forceinline__ void spmv_chunks(const DevMatrix &A, long long c0, long long c1, int row0,
-                                            const double *__restrict__ x64, const float *__restrict__ x32,
-                                            double *q)
+// Chunks per 16-lane slot and batch of the SpMV for a kernel holding RPT rows per
+// thread (bounded by the registers the pipeline needs next to the row vectors).
+template <int RPT>
+constexpr int spmv_unroll() { return RPT >= 8 ? 4 : 8; }
+
+template <int CLS>
+__device__ __forceinline__ auto load_value(const DevMatrix &A, long long e)
 {
+    if constexpr (CLS == 0) return A.v64[e];
+    else if constexpr (CLS == 1) return A.v32[e];
+    else if constexpr (CLS == 2) return A.v16[e];
+    else return A.v8[e];
+}
+
+// Class-CLS chunks [c0, c1) (headers sorted by class): q[row - row0] += a * x.
+//   CLS 0: binary64 values times x64, product in binary64.
+//   CLS 1/2/3: binary32/binary16/int8 values times x32, product in binary32, added in
+//   binary64.
+//   DYN: the chunk's class is lowered to its column tile's dynamic class (A.dvis) when
+//   that is lower, reading the value copy of that precision; dynamic class 4 skips it.
+// Three-stage software pipeline over batches of 4U chunks per wave: while batch j's
+// x gathers are in flight, batch j+1's entry indices and values (its headers arrived
+// one batch earlier) and batch j+2's headers are loaded, so a batch costs about one
+// gather latency instead of three dependent memory latencies.  All loads are unconditional (the
+// header index is clamped, the per-entry arrays carry 16 entries of padding), so no
+// load sits inside a branch; only the LDS accumulation is predicated.
+template <int CLS, int U, bool DYN>
+__device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, long long c1, int row0,
+                                           const double *__restrict__ x64, const float *__restrict__ x32,
+                                           double *q, double *qdummy)
+{
+    if (c0 >= c1) return;
+    using VT = decltype(load_value<CLS>(A, 0));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int slot = lane >> 4, k = lane & 15;
-    const long long step = (long long)nwv * 4 * kUnroll;
-    for (long long cb = c0 + (long long)wv * 4 * kUnroll; cb < c1; cb += step) {
-        int4 m[kUnroll];
+    const int k = lane & 15, slot = lane >> 4;
+    const long long step = (long long)nwv * 4 * U;
+    const long long first = c0 + (long long)wv * 4 * U + slot;
+    const long long last = c1 - 1;
+    // pipeline registers: headers of batches j and j+1 (h0, h1), entry index and value
+    // of batch j (id0, val0)
+    int3 h0[U], h1[U];
+    int id0[U];
+    VT val0[U];
+    bool v0[U], v1[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            long long c = cb + 4 * u + slot;
-            m[u] = c < c1 ? A.chunks[c] : make_int4(0, 0, 0, -1);
+    for (int u = 0; u < U; u++) {
+        const long long c = first + 4 * u;
+        v0[u] = c <= last;
+        h0[u] = A.chunks[c <= last ? c : last];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const long long c = first + step + 4 * u;
+        v1[u] = c <= last;
+        h1[u] = A.chunks[c <= last ? c : last];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        id0[u] = A.eidx[(long long)h0[u].x + k];
+        val0[u] = load_value<CLS>(A, (long long)h0[u].x + k);
+    }
+
+    for (long long cb = first; cb - slot <= last; cb += step) {
+        // stage C, batch j: x gathers
+        double xg64[U];
+        float xg32[U];
+        int qi[U], dcls[U];
+        long long ent[U];
+        (void)xg64;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool act = v0[u] && k <= chunk_last(h0[u]);
+            ent[u] = (long long)h0[u].x + k;
+            const int col = act ? chunk_colbase(h0[u]) + (id0[u] & 15) : 0;
+            qi[u] = act ? chunk_rowbase(h0[u]) - row0 + (id0[u] >> 4) : -1;
+            dcls[u] = DYN ? (int)A.dvis[chunk_coltile(h0[u])] : 0;
+            if constexpr (CLS == 0) xg64[u] = x64[col];
+            if constexpr (CLS != 0 || DYN) xg32[u] = x32[col];
         }
-        int cls[kUnroll];
+        // stage B, batch j+1: entry indices and values (its headers arrived a batch ago)
+        int id1[U];
+        VT val1[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            cls[u] = (m[u].w >> 4) & 3;
-            if (DYN && m[u].w >= 0) cls[u] = max(cls[u], (int)A.dvis[m[u].y >> 4]);
+        for (int u = 0; u < U; u++) {
+            id1[u] = A.eidx[(long long)h1[u].x + k];
+            val1[u] = load_value<CLS>(A, (long long)h1[u].x + k);
         }
-        int id[kUnroll];
+        // products of batch j (before its headers are replaced)
+        double prod[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            bool act = m[u].w >= 0 && k <= (m[u].w & 15) && (!DYN || cls[u] < 4);
-            id[u] = act ? (int)A.eidx[(long long)m[u].x + k] : -1;
+        for (int u = 0; u < U; u++) {
+            if constexpr (CLS == 0) prod[u] = val0[u] * xg64[u];
+            else prod[u] = (double)((float)val0[u] * xg32[u]);
         }
-        double prod[kUnroll];
+        // stage A, batch j+2: headers
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            prod[u] = 0.0;
-            if (id[u] >= 0) {
-                const long long e = (long long)m[u].x + k;
-                const int col = m[u].y + (id[u] & 15);
-                if (!MIXED || cls[u] == 0) {
-                    prod[u] = A.v64[e] * x64[col];
-                } else {
-                    prod[u] = (double)(val_lowp(A, cls[u], e) * x32[col]);
+        for (int u = 0; u < U; u++) {
+            const long long c = cb + 2 * step + 4 * u;
+            h0[u] = h1[u];
+            v0[u] = v1[u];
+            v1[u] = c <= last;
+            h1[u] = A.chunks[c <= last ? c : last];
+        }
+        if constexpr (DYN) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (qi[u] >= 0 && dcls[u] > CLS) {
+                    // lowered below the storage class: the value copy at that precision
+                    if (dcls[u] >= 4) qi[u] = -1;
+                    else prod[u] = (double)(val_lowp(A, dcls[u], ent[u]) * xg32[u]);
                 }
             }
         }
+        // inactive slots add 0.0 to a lane-private dummy word (no branch, no conflict)
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (id[u] >= 0) atomicAdd(&q[m[u].z - row0 + (id[u] >> 4)], prod[u]);
+        for (int u = 0; u < U; u++)
+            atomicAdd(qi[u] >= 0 ? &q[qi[u]] : &qdummy[lane], qi[u] >= 0 ? prod[u] : 0.0);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            id0[u] = id1[u];
+            val0[u] = val1[u];
+        }
+    }
+}
+
+// q[row - row0] += (A x)[row] over one range's chunks; cr[0..4] are its class bounds.
+//   MIXED = false: every chunk as class 0 (binary64 values, x64).
+//   MIXED = true : each class with its own loop.
+//   qdummy: 64 words of LDS that inactive lanes add 0.0 to (any LDS words whose value
+//   may not change, e.g. a reduction scratch area).
+template <bool MIXED, int U = 4, bool DYN = false>
+__device__ __forceinline__ void spmv_chunks(const DevMatrix &A, const long long *cr, int row0,
+                                            const double *__restrict__ x64, const float *__restrict__ x32,
+                                            double *q, double *qdummy)
+{
+    if constexpr (!MIXED) {
+        spmv_class<0, U, false>(A, cr[0], cr[4], row0, x64, x32, q, qdummy);
+    } else {
+        spmv_class<0, U, DYN>(A, cr[0], cr[1], row0, x64, x32, q, qdummy);
+        spmv_class<1, U, DYN>(A, cr[1], cr[2], row0, x64, x32, q, qdummy);
+        spmv_class<2, U, DYN>(A, cr[2], cr[3], row0, x64, x32, q, qdummy);
+        spmv_class<3, U, DYN>(A, cr[3], cr[4], row0, x64, x32, q, qdummy);
     }
 }
 

@@ -20,7 +20,7 @@ namespace mf {
 // 1..8 rows per thread.
 inline bool shape_for(int rows_cap, int &rpt, int &blk)
 {
-    for (int r = 1; r <= 8; r *= 2)
+    for (int r = 1; r * 1024 <= mfk::kMaxRowsPerWg; r *= 2)
         if (r * 1024 >= rows_cap) {
             rpt = r;
             blk = 1024;

@@ -3,10 +3,12 @@
 // standalone SpMV.
 //
 // Device layout (HBM), derived from the Tile_create order:
-//   chunks  int4[C]  16-entry pieces of tiles {e0, colbase, rowbase, (cnt-1)|class<<4}
-//   eidx    u8[nnz]  (local row << 4) | local column of every entry
-//   v64     f64[nnz] all values;  v32/v16/v8: the values of class 1/2/3 tiles at that
-//                    precision (only those entries are ever read)
+//   chunks  int3[C]    16-entry pieces of tiles {e0, column tile | class << 28,
+//                      tile row | (cnt-1) << 27}: entries [e0, e0+cnt)
+//   eidx    u8[nnz]    (local row << 4) | local column, in tile order
+//   v64     f64[nnz]   all values; v32/v16/v8 hold every value at that precision (the
+//                      tile's own class, or the copies read under dynamic classes)
+//   src     i32[nnz]   CSR position of each entry
 // A tile row's chunks are contiguous, so a workgroup owning a contiguous range of
 // tile rows owns a contiguous range of chunks.
 #include <hip/hip_runtime.h>
@@ -46,6 +48,16 @@ mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s)
 
 DeviceLayout::~DeviceLayout() { release(); }
 
+void free_partition(Partition &P)
+{
+    free_dev(P.wg_tr);
+    free_dev(P.wg_chunk);
+    free_dev(P.chunks);
+    P.wg_tr = nullptr;
+    P.wg_chunk = nullptr;
+    P.chunks = nullptr;
+}
+
 void DeviceLayout::release()
 {
     free_dev(chunks);
@@ -55,10 +67,7 @@ void DeviceLayout::release()
     free_dev(v16);
     free_dev(v8);
     free_dev(src);
-    for (auto &kv : parts) {
-        free_dev(kv.second.wg_tr);
-        free_dev(kv.second.wg_chunk);
-    }
+    for (auto &kv : parts) free_partition(kv.second);
     parts.clear();
     chunks = nullptr;
     eidx = nullptr;
@@ -89,9 +98,13 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     }
     const int64_t C = L.tr_chunk[ts.tilem];
     L.nchunks = C;
-    std::vector<int4> ch(C > 0 ? C : 1);
-    std::vector<uint8_t> eid(ts.nnz > 0 ? ts.nnz : 1);
-    std::vector<double> v64(ts.nnz > 0 ? ts.nnz : 1);
+    // 16 zero entries of padding, so a chunk's 16 lanes may load unconditionally
+    const int64_t P = ts.nnz + 16;
+    std::vector<int3> &ch = L.h_chunks;
+    ch.assign(C > 0 ? C : 1, make_int3(0, 0, -1));
+    std::vector<uint8_t> eid(P, 0);
+    std::vector<double> v64(P, 0.0);
+    std::vector<int> src32(P, -1);
     std::vector<float> v32;
     std::vector<uint16_t> v16;
     std::vector<int8_t> v8;
@@ -103,9 +116,9 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
         if (cls[t] != MF_FP64) any_lowp = true;
     }
     if (keep_mixed) {
-        v32.assign(ts.nnz, 0.f);
-        v16.assign(ts.nnz, 0);
-        v8.assign(ts.nnz, 0);
+        v32.assign(P, 0.f);
+        v16.assign(P, 0);
+        v8.assign(P, 0);
     }
 #pragma omp parallel for schedule(dynamic, 64)
     for (int bi = 0; bi < ts.tilem; bi++) {
@@ -113,13 +126,15 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
         for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++) {
             const int64_t a = ts.tile_off[t], b = ts.tile_off[t + 1];
             for (int64_t e0 = a; e0 < b; e0 += 16, c++) {
-                int cnt = (int)std::min<int64_t>(16, b - e0);
-                ch[c] = make_int4((int)e0, ts.tile_col[t] * kTile, bi * kTile, (cnt - 1) | (cls[t] << 4));
+                const int cnt = (int)std::min<int64_t>(16, b - e0);
+                ch[c] = make_int3((int)e0, (int)((unsigned)ts.tile_col[t] | ((unsigned)cls[t] << 28)),
+                                  (int)((unsigned)bi | ((unsigned)(cnt - 1) << 27)));
             }
             for (int64_t e = a; e < b; e++) {
                 eid[e] = (uint8_t)((ts.lrow[e] << 4) | ts.lcol[e]);
-                double v = val[ts.src[e]];
+                const double v = val[ts.src[e]];
                 v64[e] = v;
+                src32[e] = (int)ts.src[e];
                 if (!v32.empty()) {
                     // the tile's own class holds its rounding of the value; the lower
                     // precisions hold the copies of main-cg-mixed.cu:2333-2350 (f32 RNE,
@@ -141,18 +156,13 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     L.has_copies = !v32.empty();
     L.need_p64_mixed = L.tiles_by_class[MF_FP64] > 0;
     mf_status st;
-    if ((st = upload((void **)&L.chunks, ch.data(), sizeof(int4) * C, s)) != MF_SUCCESS) return st;
-    if ((st = upload((void **)&L.eidx, eid.data(), ts.nnz, s)) != MF_SUCCESS) return st;
-    if ((st = upload((void **)&L.v64, v64.data(), sizeof(double) * ts.nnz, s)) != MF_SUCCESS) return st;
-    {
-        std::vector<int> src32(ts.nnz > 0 ? ts.nnz : 1);
-        for (int64_t e = 0; e < ts.nnz; e++) src32[e] = (int)ts.src[e];
-        if ((st = upload((void **)&L.src, src32.data(), sizeof(int) * ts.nnz, s)) != MF_SUCCESS) return st;
-    }
+    if ((st = upload((void **)&L.eidx, eid.data(), P, s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&L.v64, v64.data(), sizeof(double) * P, s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&L.src, src32.data(), sizeof(int) * P, s)) != MF_SUCCESS) return st;
     if (L.has_copies) {
-        if ((st = upload((void **)&L.v32, v32.data(), sizeof(float) * ts.nnz, s)) != MF_SUCCESS) return st;
-        if ((st = upload((void **)&L.v16, v16.data(), sizeof(uint16_t) * ts.nnz, s)) != MF_SUCCESS) return st;
-        if ((st = upload((void **)&L.v8, v8.data(), ts.nnz, s)) != MF_SUCCESS) return st;
+        if ((st = upload((void **)&L.v32, v32.data(), sizeof(float) * P, s)) != MF_SUCCESS) return st;
+        if ((st = upload((void **)&L.v16, v16.data(), sizeof(uint16_t) * P, s)) != MF_SUCCESS) return st;
+        if ((st = upload((void **)&L.v8, v8.data(), P, s)) != MF_SUCCESS) return st;
     }
     if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
     // algorithmic bytes of one pass over the value stream (12 B per tile + value + 1 B column)
@@ -160,7 +170,7 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     L.alg_bytes_f64 = 12.0 * (double)T + 9.0 * (double)ts.nnz;
     L.alg_bytes_mixed = 12.0 * (double)T;
     for (int c = 0; c < 4; c++) L.alg_bytes_mixed += (sv[c] + 1.0) * (double)L.nnz_by_class[c];
-    L.device_bytes = (double)C * 16 + (double)ts.nnz * (1 + 8 + 4 + (L.has_copies ? 7 : 0));
+    L.device_bytes = (double)C * 12 + (double)P * (1 + 8 + 4 + (L.has_copies ? 7 : 0));
     return MF_SUCCESS;
 }
 
@@ -173,42 +183,81 @@ mf_status DeviceLayout::partition(int G, hipStream_t s, const Partition **out)
         *out = &it->second;
         return MF_SUCCESS;
     }
-    Partition P;
-    P.G = G;
-    std::vector<int> tr(G + 1, 0);
-    std::vector<long long> cc(G + 1, 0);
-    const double total = (double)nchunks + (double)tilem;
     if (G > tilem) return MF_ERR_INVALID;
+    std::vector<int> tr(G + 1, 0);
+    const double total = (double)nchunks + (double)tilem;
+    // Cap the tile rows of one workgroup at what a 1024-thread workgroup holds in
+    // registers (8 rows per thread) whenever the grid can cover the matrix that way, so
+    // a balanced split never pushes one workgroup over the persistent kernels' shape.
+    const int cap_tr =
+        (long long)G * mfk::kMaxRowsPerWg / kTile >= tilem ? mfk::kMaxRowsPerWg / kTile : tilem;
     int bi = 0;
     for (int w = 1; w < G; w++) {
         const double target = total * w / G;
         while (bi < tilem && (double)tr_chunk[bi + 1] + (bi + 1) <= target) bi++;
-        // every workgroup keeps at least one tile row
+        // every workgroup keeps at least one tile row and at most cap_tr; the rest
+        // must stay coverable by the remaining workgroups
         bi = std::min(std::max(bi, tr[w - 1] + 1), tilem - (G - w));
+        bi = std::min(bi, tr[w - 1] + cap_tr);
+        bi = std::max(bi, (int)std::max<long long>(0, (long long)tilem - (long long)(G - w) * cap_tr));
         tr[w] = bi;
     }
     tr[G] = tilem;
-    int rows_cap = 0;
-    for (int w = 0; w < G; w++) {
-        cc[w] = tr_chunk[tr[w]];
-        int r0 = tr[w] * kTile, r1 = std::min(tr[w + 1] * kTile, n_rows);
-        rows_cap = std::max(rows_cap, std::max(0, r1 - r0));
+    Partition P;
+    mf_status st = build_ranges(tr, s, P);
+    if (st != MF_SUCCESS) {
+        free_partition(P);
+        return st;
     }
-    cc[G] = tr_chunk[tilem];
-    P.rows_cap = rows_cap;
-    mf_status st;
-    if ((st = upload((void **)&P.wg_tr, tr.data(), sizeof(int) * (G + 1), s)) != MF_SUCCESS) return st;
-    if ((st = upload((void **)&P.wg_chunk, cc.data(), sizeof(long long) * (G + 1), s)) != MF_SUCCESS) return st;
-    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
     auto res = parts.emplace(G, P);
     *out = &res.first->second;
+    return MF_SUCCESS;
+}
+
+mf_status DeviceLayout::build_ranges(const std::vector<int> &cuts, hipStream_t s, Partition &P) const
+{
+    const int R = (int)cuts.size() - 1;
+    if (R < 1 || cuts[0] != 0 || cuts[R] != tilem) return MF_ERR_INVALID;
+    P.G = R;
+    std::vector<long long> cb(4 * (size_t)R + 1, 0);
+    std::vector<int3> sorted(nchunks > 0 ? nchunks : 1, make_int3(0, 0, -1));
+    // count per (range, class), then place: ranges are independent
+    std::vector<long long> cnt(4 * (size_t)R, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int r = 0; r < R; r++)
+        for (int64_t c = tr_chunk[cuts[r]]; c < tr_chunk[cuts[r + 1]]; c++)
+            cnt[4 * (size_t)r + (((unsigned)h_chunks[c].y >> 28) & 3)]++;
+    long long pos = 0;
+    for (size_t i = 0; i < 4 * (size_t)R; i++) {
+        cb[i] = pos;
+        pos += cnt[i];
+    }
+    cb[4 * (size_t)R] = pos;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int r = 0; r < R; r++) {
+        long long at[4];
+        for (int k = 0; k < 4; k++) at[k] = cb[4 * (size_t)r + k];
+        for (int64_t c = tr_chunk[cuts[r]]; c < tr_chunk[cuts[r + 1]]; c++)
+            sorted[at[((unsigned)h_chunks[c].y >> 28) & 3]++] = h_chunks[c];
+    }
+    int rows_cap = 0;
+    for (int r = 0; r < R; r++) {
+        const int r0 = cuts[r] * kTile, r1 = std::min(cuts[r + 1] * kTile, n_rows);
+        rows_cap = std::max(rows_cap, std::max(0, r1 - r0));
+    }
+    P.rows_cap = rows_cap;
+    mf_status st;
+    if ((st = upload((void **)&P.wg_tr, cuts.data(), sizeof(int) * (R + 1), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&P.wg_chunk, cb.data(), sizeof(long long) * cb.size(), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&P.chunks, sorted.data(), sizeof(int3) * sorted.size(), s)) != MF_SUCCESS) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
     return MF_SUCCESS;
 }
 
 mfk::DevMatrix DeviceLayout::dev(const Partition &P) const
 {
     mfk::DevMatrix d;
-    d.chunks = chunks;
+    d.chunks = P.chunks;
     d.eidx = eidx;
     d.v64 = v64;
     d.v32 = v32;
@@ -289,12 +338,13 @@ __global__ __launch_bounds__(1024) void spmv_kernel(DevMatrix A, const double *_
                                                     const float *__restrict__ x32, double *__restrict__ y)
 {
     extern __shared__ double q[];
+    __shared__ double qdummy[64];
     const int w = blockIdx.x;
     const int tr0 = A.wg_tr[w], tr1 = A.wg_tr[w + 1];
     const int row0 = tr0 * kTile, row1 = min(tr1 * kTile, A.n);
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) q[i] = 0.0;
     __syncthreads();
-    spmv_chunks<MIXED>(A, A.wg_chunk[w], A.wg_chunk[w + 1], row0, x64, x32, q);
+    spmv_chunks<MIXED, 4>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
     __syncthreads();
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
 }
@@ -307,11 +357,11 @@ __global__ void pattern_outer_kernel(DevMatrix A, const int *__restrict__ src, l
     const long long c = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 4;
     const int k = threadIdx.x & 15;
     if (c >= nchunks) return;
-    const int4 m = A.chunks[c];
-    if (k > (m.w & 15)) return;
-    const long long e = (long long)m.x + k;
+    const int3 h = A.chunks[c];
+    if (k > chunk_last(h)) return;
+    const long long e = (long long)h.x + k;
     const int id = A.eidx[e];
-    out[src[e] - base] = scale * u[m.z + (id >> 4)] * v[m.y + (id & 15)];
+    out[src[e] - base] = scale * u[chunk_rowbase(h) + (id >> 4)] * v[chunk_colbase(h) + (id & 15)];
 }
 
 // Dynamic class of every 16-element segment (the classification used inside the

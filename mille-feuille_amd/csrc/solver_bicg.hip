@@ -58,7 +58,7 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
 
 // r <- b - A x (binary64 stream); returns grid {r'r, rhat'r}.
 template <int RPT, int BLK>
-__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, long long c0, long long c1, int wg,
+__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int wg,
                                    double *q, double *xs, double *scratch, unsigned &epoch, double (&r)[RPT],
                                    double &rr, double &rho)
 {
@@ -70,7 +70,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, long lo
     if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
     __syncthreads();
-    spmv_chunks<false, (RPT >= 8 ? 2 : 4)>(a.A, c0, c1, R.row0, a.x, nullptr, q);
+    spmv_chunks<false, spmv_unroll<RPT>()>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
     __syncthreads();
     double part[2] = {0.0, 0.0};
     R.refresh();
@@ -102,15 +102,15 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     R.row0 = tr0 * kTile;
     R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
     R.refresh();
-    const long long c0 = a.A.wg_chunk[wg], c1 = a.A.wg_chunk[wg + 1];
-    constexpr int U = RPT >= 8 ? 2 : 4;
+    const long long *cr = a.A.wg_chunk + 4 * (size_t)wg;
+    constexpr int U = spmv_unroll<RPT>();
     unsigned epoch = 0;
 
     double r[RPT], p[RPT], v[RPT], sv[RPT];
     if (a.use_x0) {
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<false, U>(a.A, c0, c1, R.row0, a.x, nullptr, q);
+        spmv_chunks<false, U>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
         __syncthreads();
     }
     double part[2] = {0.0, 0.0};
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         if (!a.fixed) {
             bool conv = rr <= a.tol2 * bb;
             if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                if (!bicg_true_residual<RPT, BLK>(a, R, c0, c1, wg, q, xs, scratch, epoch, r, rr, rho)) {
+                if (!bicg_true_residual<RPT, BLK>(a, R, cr, wg, q, xs, scratch, epoch, r, rr, rho)) {
                     status = BST_TIMEOUT;
                     break;
                 }
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         // v = A p ; alpha = rho / rhat'v
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, c0, c1, R.row0, a.p64, a.p32, q);
+        spmv_chunks<MIXED, U>(a.A, cr, R.row0, a.p64, a.p32, q, scratch);
         __syncthreads();
         double d1[1] = {0.0};
         R.refresh();
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         // t = A s ; omega = t's / t't ; rho' = rhat's - omega rhat't
         zero_q<RPT, BLK>(q, R);
         __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, c0, c1, R.row0, a.s64, a.s32, q);
+        spmv_chunks<MIXED, U>(a.A, cr, R.row0, a.s64, a.s32, q, scratch);
         __syncthreads();
         double d4[4] = {0.0, 0.0, 0.0, 0.0};
         R.refresh();
@@ -256,7 +256,7 @@ done:
     if (status != BST_TIMEOUT) {
         double rtr[RPT];
         double rr_t = 0.0, rho_t = 0.0;
-        if (bicg_true_residual<RPT, BLK>(a, R, c0, c1, wg, q, xs, scratch, epoch, rtr, rr_t, rho_t)) rt = rr_t;
+        if (bicg_true_residual<RPT, BLK>(a, R, cr, wg, q, xs, scratch, epoch, rtr, rr_t, rho_t)) rt = rr_t;
         else status = BST_TIMEOUT;
     } else {
         R.refresh();

@@ -12,18 +12,26 @@
 
 namespace mf {
 
+// Contiguous tile-row ranges [wg_tr[r], wg_tr[r+1]) and, for each range, its chunk
+// headers sorted by storage class: class k chunks of range r are
+// chunks[wg_chunk[4r+k] .. wg_chunk[4r+k+1]), so the SpMV runs one branch-free loop
+// per class.
 struct Partition {
-    int G = 0;
+    int G = 0;                      // number of ranges
     int rows_cap = 0;
     int *wg_tr = nullptr;           // device, G+1
-    long long *wg_chunk = nullptr;  // device, G+1
+    long long *wg_chunk = nullptr;  // device, 4G+1
+    int3 *chunks = nullptr;         // device, class-sorted headers
 };
+
+void free_partition(Partition &P);
 
 struct DeviceLayout {
     int n_rows = 0, n_cols = 0, tilem = 0;
     int64_t nnz = 0, tiles = 0, nchunks = 0;
     std::vector<int64_t> tr_chunk;  // host, tilem+1
-    int4 *chunks = nullptr;
+    std::vector<int3> h_chunks;     // host copy of the headers in tile order
+    int3 *chunks = nullptr;
     uint8_t *eidx = nullptr;
     double *v64 = nullptr;
     float *v32 = nullptr;
@@ -44,6 +52,9 @@ struct DeviceLayout {
     ~DeviceLayout();
     void release();
     mf_status partition(int G, hipStream_t s, const Partition **out);
+    // Build a Partition over the given tile-row cut points (cuts[0] = 0, cuts.back() =
+    // tilem, non-decreasing).  The caller owns the result (free_partition).
+    mf_status build_ranges(const std::vector<int> &cuts, hipStream_t s, Partition &P) const;
     mfk::DevMatrix dev(const Partition &P) const;
 };
 
