@@ -36,6 +36,80 @@ def cpu_model():
     return platform.processor()
 
 
+def run_extras(mf, torch, dev, log):
+    """Batched CG over 512 SPD systems (one persistent hipGraph launch vs 512 single-system
+    launches) and the differentiable solve on 64^3 variable-coefficient diffusion."""
+    out = {}
+    systems = mf.workloads.batched_poisson(512, seed=3)
+    B = mf.Batch([(rp, ci, va) for rp, ci, va, _ in systems], class_policy="lossless")
+    bp = B.pack([b for *_, b in systems])
+    x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")   # captures the graph
+    reps = 3
+    t_b = []
+    for _ in range(reps):
+        x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")
+        t_b.append(ms)
+    xs = B.unpack(x)
+    worst = 0.0
+    for (rp, ci, va, b), xv in zip(systems, xs):
+        r = b - mf.workloads.spmv(rp, ci, va, xv)
+        worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(b)))
+    conv = sum(1 for i in infos if i["status"] == 0)
+    B.close()
+    # 512 sequential single-system launches of the same solver
+    mats = [mf.TiledMatrix(rp, ci, va, class_policy="lossless") for rp, ci, va, _ in systems]
+    bs = [torch.as_tensor(b, device=dev) for *_, b in systems]
+    for M, bb in zip(mats[:8], bs[:8]):
+        M.cg(bb, tol=1e-10, precision="mixed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_kernel = 0.0
+    for M, bb in zip(mats, bs):
+        _, inf = M.cg(bb, tol=1e-10, precision="mixed")
+        seq_kernel += inf["time_ms"]
+    torch.cuda.synchronize()
+    seq_wall = (time.perf_counter() - t0) * 1e3
+    for M in mats:
+        M.close()
+    tb = float(np.median(t_b))
+    out["batched_512"] = {"systems": len(systems), "converged": conv, "worst_relres_true_host": worst,
+                          "launch_ms": tb, "solves_per_s": len(systems) / (tb * 1e-3),
+                          "sequential_kernel_ms": seq_kernel, "sequential_wall_ms": seq_wall,
+                          "sequential_solves_per_s": len(systems) / (seq_wall * 1e-3),
+                          "speedup_vs_sequential_wall": seq_wall / tb}
+    log(f"[bench] batched 512: {tb:.2f} ms ({conv} converged, worst {worst:.2e}); "
+        f"sequential {seq_wall:.1f} ms wall")
+    # differentiable solve, 64^3 variable-coefficient diffusion
+    SparseOperator, sparse_solve = mf.autograd.SparseOperator, mf.autograd.sparse_solve
+    rp, ci, va, _k = mf.workloads.varcoef_diffusion3d(64, seed=4)
+    n = rp.size - 1
+    op = SparseOperator(rp, ci, va, spd=True, precision="mixed", class_policy="rowscaled")
+    rng = np.random.default_rng(4)
+    xt = torch.as_tensor(rng.uniform(-1, 1, n), device=dev)
+    bvec = torch.as_tensor(rng.uniform(-1, 1, n), device=dev)
+    vals = torch.as_tensor(va, device=dev).requires_grad_(True)
+    bt = bvec.clone().requires_grad_(True)
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xsol = sparse_solve(vals, bt, op)
+        loss = 0.5 * torch.sum((xsol - xt) ** 2)
+        loss.backward()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        vals.grad = None
+        bt.grad = None
+    fi, ai = op.last_info.get("forward", {}), op.last_info.get("adjoint", {})
+    out["autograd_64cubed"] = {"n": n, "nnz": int(ci.size), "forward_backward_ms": float(np.median(times)),
+                               "forward_iterations": fi.get("iterations"), "adjoint_iterations": ai.get("iterations"),
+                               "forward_kernel_ms": fi.get("time_ms"), "adjoint_kernel_ms": ai.get("time_ms")}
+    log(f"[bench] autograd 64^3: forward+backward {np.median(times):.2f} ms "
+        f"(it {fi.get('iterations')} + {ai.get('iterations')})")
+    op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,7 +118,12 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bicg", action="store_true")
+    ap.add_argument("--extras", action="store_true",
+                    help="also measure the 512-system batched solve and the 64^3 differentiable solve")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (reference kernel of known "
+                         "traffic for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -52,10 +131,19 @@ def main():
         sys.exit(2)
 
     import torch
-    from __graft_entry__ import load_package
-    mf = load_package()
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 3}), flush=True)
+        return
+    from __graft_entry__ import load_package
+    mf = load_package()
     nx = args.grid
 
     def log(msg):
@@ -138,6 +226,9 @@ def main():
         log(f"[bench] BiCGSTAB convdiff: mixed {tb_m:.3f} ms ({ib_m[-1]['iterations']} it) "
             f"fp64 {tb_d:.3f} ms ({ib_d[-1]['iterations']} it)")
         A2.close()
+
+    if args.extras:
+        extra.update(run_extras(mf, torch, dev, log))
 
     cpu = None
     if not args.no_cpu:

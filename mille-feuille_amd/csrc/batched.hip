@@ -383,10 +383,10 @@ template <bool MIXED>
 static BatchKernel pick_batch(int rpt)
 {
     switch (rpt) {
-    case 1: return mfk::batched_cg_kernel<1, MIXED, mfk::spmv_unroll<1>()>;
-    case 2: return mfk::batched_cg_kernel<2, MIXED, mfk::spmv_unroll<2>()>;
-    case 4: return mfk::batched_cg_kernel<4, MIXED, mfk::spmv_unroll<4>()>;
-    case 8: return mfk::batched_cg_kernel<8, MIXED, mfk::spmv_unroll<8>()>;
+    case 1: return mfk::batched_cg_kernel<1, MIXED, mfk::spmv_unroll<1, 3>()>;
+    case 2: return mfk::batched_cg_kernel<2, MIXED, mfk::spmv_unroll<2, 3>()>;
+    case 4: return mfk::batched_cg_kernel<4, MIXED, mfk::spmv_unroll<4, 3>()>;
+    case 8: return mfk::batched_cg_kernel<8, MIXED, mfk::spmv_unroll<8, 3>()>;
     default: return nullptr;
     }
 }

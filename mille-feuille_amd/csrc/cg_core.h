@@ -132,7 +132,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
     __syncthreads();
-    spmv_chunks<false, spmv_unroll<RPT>()>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
+    spmv_chunks<false, spmv_unroll<RPT, XREG ? 3 : 2>()>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
     __syncthreads();
     double part[1] = {0.0};
     R.refresh();
@@ -152,7 +152,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
 
 // The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
 // a.x; every member returns the same CgResult.
-template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = spmv_unroll<RPT>(), bool XREG = false,
+template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = spmv_unroll<RPT, 2>(), bool XREG = false,
           bool DYN = false>
 __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
                             double *xs_lds)
@@ -423,7 +423,8 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.dvis = a.dvis;
     part.dyn_mode = a.dyn_mode;
     part.n_sys = a.A.n;
-    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT>(), XREG, DYN>(part, R, sy, scratch, q, xs);
+    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT, (XREG || DYN) ? 3 : 2>(), XREG, DYN>(
+        part, R, sy, scratch, q, xs);
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)res.it;
         a.info[1] = res.rr;

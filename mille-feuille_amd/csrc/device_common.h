@@ -161,18 +161,146 @@ __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long
     return (float)A.v8[e];
 }
 
-// Chunks per 16-lane slot and batch of the SpMV for a kernel holding RPT rows per
-// thread (bounded by the registers the pipeline needs next to the row vectors).
-template <int RPT>
-constexpr int spmv_unroll() { return RPT >= 8 ? 4 : 8; }
+// Chunk groups (16 chunks per wave each) per batch of the SpMV for a kernel holding VEC
+// binary64 vectors of RPT rows per thread in registers.  Two groups keep about 80 KB per
+// CU in flight across the three pipeline stages, well above what covers HBM latency;
+// one group when the row vectors already take 48+ registers, so nothing spills.
+template <int RPT, int VEC = 2>
+constexpr int spmv_unroll() { return RPT * VEC >= 24 ? 1 : 2; }
 
+// Four consecutive values starting at a multiple-of-4 position, loaded with one
+// aligned access (two for binary64), and value j of such a group.
+struct Quad64 {
+    double2 lo, hi;
+};
 template <int CLS>
-__device__ __forceinline__ auto load_value(const DevMatrix &A, long long e)
+__device__ __forceinline__ auto load_quad(const DevMatrix &A, long long e)
 {
-    if constexpr (CLS == 0) return A.v64[e];
-    else if constexpr (CLS == 1) return A.v32[e];
-    else if constexpr (CLS == 2) return A.v16[e];
-    else return A.v8[e];
+    if constexpr (CLS == 0) {
+        const double2 *p = reinterpret_cast<const double2 *>(A.v64 + e);
+        return Quad64{p[0], p[1]};
+    } else if constexpr (CLS == 1) {
+        return *reinterpret_cast<const float4 *>(A.v32 + e);
+    } else if constexpr (CLS == 2) {
+        return *reinterpret_cast<const uint2 *>(A.v16 + e);
+    } else {
+        return *reinterpret_cast<const unsigned *>(A.v8 + e);
+    }
+}
+
+template <int CLS, class Q>
+__device__ __forceinline__ auto quad_value(const Q &v, int j)
+{
+    if constexpr (CLS == 0) {
+        return j == 0 ? v.lo.x : j == 1 ? v.lo.y : j == 2 ? v.hi.x : v.hi.y;
+    } else if constexpr (CLS == 1) {
+        return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+    } else if constexpr (CLS == 2) {
+        const unsigned w = j < 2 ? v.x : v.y;
+        const unsigned short b = (unsigned short)(w >> (16 * (j & 1)));
+        return (float)__builtin_bit_cast(_Float16, b);
+    } else {
+        return (float)(signed char)(v >> (8 * j));
+    }
+}
+
+// Pipeline registers of one batch of the SpMV: chunk headers, entry indices (4 per
+// lane) and values (4 per lane), and whether each chunk exists.
+template <int CLS, int U>
+struct SpmvStage {
+    using QT = decltype(load_quad<CLS>(DevMatrix{}, 0));
+    int3 h[U];
+    unsigned id[U];
+    QT val[U];
+    bool v[U];
+};
+
+template <int CLS, int U>
+__device__ __forceinline__ void spmv_load_headers(const DevMatrix &A, SpmvStage<CLS, U> &S, int cb, int last)
+{
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int c = cb + 16 * u;
+        S.v[u] = c <= last;
+        S.h[u] = A.chunks[c <= last ? c : last];
+    }
+}
+
+template <int CLS, int U>
+__device__ __forceinline__ void spmv_load_entries(const DevMatrix &A, SpmvStage<CLS, U> &S, int sub)
+{
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const long long e = (long long)S.h[u].x + 4 * sub;
+        S.id[u] = *reinterpret_cast<const unsigned *>(A.eidx + e);
+        S.val[u] = load_quad<CLS>(A, e);
+    }
+}
+
+// One pipeline step: X holds batch j (entries loaded), Y batch j+1 (headers loaded).
+// Gathers and accumulates batch j, loads Y's entries, and reloads X's headers with
+// batch j+2 (cb2 = its first chunk).
+template <int CLS, int U, bool DYN>
+__device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> &X, SpmvStage<CLS, U> &Y,
+                                          int cb2, int last, int row0, int sub, int lane,
+                                          const double *__restrict__ x64, const float *__restrict__ x32,
+                                          double *q, double *qdummy)
+{
+    double xg64[U][4];
+    float xg32[U][4];
+    int qi[U][4], dcls[U];
+    long long ebase[U];
+    (void)xg64;
+    (void)ebase;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int lastk = X.v[u] ? chunk_last(X.h[u]) : -1;
+        const int cbase = chunk_colbase(X.h[u]), rbase = chunk_rowbase(X.h[u]) - row0;
+        dcls[u] = DYN ? (int)A.dvis[chunk_coltile(X.h[u])] : 0;
+        ebase[u] = (long long)X.h[u].x + 4 * sub;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool act = 4 * sub + j <= lastk;
+            const int id = (X.id[u] >> (8 * j)) & 255;
+            const int col = act ? cbase + (id & 15) : 0;
+            qi[u][j] = act ? rbase + (id >> 4) : -1;
+            if constexpr (CLS == 0) xg64[u][j] = x64[col];
+            if constexpr (CLS != 0 || DYN) xg32[u][j] = x32[col];
+        }
+    }
+    spmv_load_entries<CLS, U>(A, Y, sub);
+    double prod[U][4];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if constexpr (CLS == 0) prod[u][j] = quad_value<CLS>(X.val[u], j) * xg64[u][j];
+            else prod[u][j] = (double)(quad_value<CLS>(X.val[u], j) * xg32[u][j]);
+        }
+    if constexpr (DYN) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (dcls[u] > CLS) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (qi[u][j] < 0) continue;
+                    // lowered below the storage class: the value copy at that precision
+                    if (dcls[u] >= 4) qi[u][j] = -1;
+                    else prod[u][j] = (double)(val_lowp(A, dcls[u], ebase[u] + j) * xg32[u][j]);
+                }
+            }
+        }
+    }
+    spmv_load_headers<CLS, U>(A, X, cb2, last);
+    // inactive entries add 0.0 to a lane-private dummy word; the product is masked by
+    // multiplication so that it is computed (and its loads issued) unconditionally
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool act = qi[u][j] >= 0;
+            atomicAdd(act ? &q[qi[u][j]] : &qdummy[lane], prod[u][j] * (act ? 1.0 : 0.0));
+        }
 }
 
 // Class-CLS chunks [c0, c1) (headers sorted by class): q[row - row0] += a * x.
@@ -181,109 +309,41 @@ __device__ __forceinline__ auto load_value(const DevMatrix &A, long long e)
 //   binary64.
 //   DYN: the chunk's class is lowered to its column tile's dynamic class (A.dvis) when
 //   that is lower, reading the value copy of that precision; dynamic class 4 skips it.
-// Three-stage software pipeline over batches of 4U chunks per wave: while batch j's
-// x gathers are in flight, batch j+1's entry indices and values (its headers arrived
-// one batch earlier) and batch j+2's headers are loaded, so a batch costs about one
-// gather latency instead of three dependent memory latencies.  All loads are unconditional (the
-// header index is clamped, the per-entry arrays carry 16 entries of padding), so no
-// load sits inside a branch; only the LDS accumulation is predicated.
+// Four lanes share a chunk, each owning 4 consecutive entries: one 4-byte load of entry
+// indices and one aligned 4-value load per lane, so a wave instruction covers 16
+// chunks.  Three-stage software pipeline over batches of 16U chunks per wave: while
+// batch j's x gathers are in flight, batch j+1's indices and values (its headers
+// arrived a batch earlier) and batch j+2's headers are loaded.  Two register sets
+// alternate (the loop body is unrolled twice), so no register with a load in flight is
+// ever copied; every load is unconditional (clamped header index, padded per-entry
+// arrays), so the loop has no branch for the compiler to sink a load into.
 template <int CLS, int U, bool DYN>
 __device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, long long c1, int row0,
                                            const double *__restrict__ x64, const float *__restrict__ x32,
                                            double *q, double *qdummy)
 {
     if (c0 >= c1) return;
-    using VT = decltype(load_value<CLS>(A, 0));
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int k = lane & 15, slot = lane >> 4;
-    const long long step = (long long)nwv * 4 * U;
-    const long long first = c0 + (long long)wv * 4 * U + slot;
-    const long long last = c1 - 1;
-    // pipeline registers: headers of batches j and j+1 (h0, h1), entry index and value
-    // of batch j (id0, val0)
-    int3 h0[U], h1[U];
-    int id0[U];
-    VT val0[U];
-    bool v0[U], v1[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const long long c = first + 4 * u;
-        v0[u] = c <= last;
-        h0[u] = A.chunks[c <= last ? c : last];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const long long c = first + step + 4 * u;
-        v1[u] = c <= last;
-        h1[u] = A.chunks[c <= last ? c : last];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        id0[u] = A.eidx[(long long)h0[u].x + k];
-        val0[u] = load_value<CLS>(A, (long long)h0[u].x + k);
-    }
-
-    for (long long cb = first; cb - slot <= last; cb += step) {
-        // stage C, batch j: x gathers
-        double xg64[U];
-        float xg32[U];
-        int qi[U], dcls[U];
-        long long ent[U];
-        (void)xg64;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const bool act = v0[u] && k <= chunk_last(h0[u]);
-            ent[u] = (long long)h0[u].x + k;
-            const int col = act ? chunk_colbase(h0[u]) + (id0[u] & 15) : 0;
-            qi[u] = act ? chunk_rowbase(h0[u]) - row0 + (id0[u] >> 4) : -1;
-            dcls[u] = DYN ? (int)A.dvis[chunk_coltile(h0[u])] : 0;
-            if constexpr (CLS == 0) xg64[u] = x64[col];
-            if constexpr (CLS != 0 || DYN) xg32[u] = x32[col];
-        }
-        // stage B, batch j+1: entry indices and values (its headers arrived a batch ago)
-        int id1[U];
-        VT val1[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            id1[u] = A.eidx[(long long)h1[u].x + k];
-            val1[u] = load_value<CLS>(A, (long long)h1[u].x + k);
-        }
-        // products of batch j (before its headers are replaced)
-        double prod[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if constexpr (CLS == 0) prod[u] = val0[u] * xg64[u];
-            else prod[u] = (double)((float)val0[u] * xg32[u]);
-        }
-        // stage A, batch j+2: headers
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const long long c = cb + 2 * step + 4 * u;
-            h0[u] = h1[u];
-            v0[u] = v1[u];
-            v1[u] = c <= last;
-            h1[u] = A.chunks[c <= last ? c : last];
-        }
-        if constexpr (DYN) {
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (qi[u] >= 0 && dcls[u] > CLS) {
-                    // lowered below the storage class: the value copy at that precision
-                    if (dcls[u] >= 4) qi[u] = -1;
-                    else prod[u] = (double)(val_lowp(A, dcls[u], ent[u]) * xg32[u]);
-                }
-            }
-        }
-        // inactive slots add 0.0 to a lane-private dummy word (no branch, no conflict)
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            atomicAdd(qi[u] >= 0 ? &q[qi[u]] : &qdummy[lane], qi[u] >= 0 ? prod[u] : 0.0);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            id0[u] = id1[u];
-            val0[u] = val1[u];
-        }
-    }
+    // chunk indices fit in 32 bits (a chunk holds at least 4 of the < 2^31 positions);
+    // the wave's batch base is uniform, so the loop branches are scalar
+    const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int sub = lane & 3, cs = lane >> 2;
+    const int step = nwv * 16 * U;
+    const int last = (int)c1 - 1;
+    int bw = (int)c0 + wv * 16 * U;
+    SpmvStage<CLS, U> X, Y;
+    spmv_load_headers<CLS, U>(A, X, bw + cs, last);
+    spmv_load_headers<CLS, U>(A, Y, bw + step + cs, last);
+    spmv_load_entries<CLS, U>(A, X, sub);
+    if (bw > last) return;
+    // batches go in pairs with a single exit test after both, so no load for a later
+    // batch can be moved behind an exit branch; the second batch of the last pair may
+    // lie past the end (clamped loads, every entry inactive)
+    do {
+        spmv_step<CLS, U, DYN>(A, X, Y, bw + 2 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN>(A, Y, X, bw + 3 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        bw += 2 * step;
+    } while (bw <= last);
 }
 
 // q[row - row0] += (A x)[row] over one range's chunks; cr[0..4] are its class bounds.
@@ -296,10 +356,11 @@ __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, const long long 
                                             const double *__restrict__ x64, const float *__restrict__ x32,
                                             double *q, double *qdummy)
 {
+    constexpr int U0 = U > 1 ? U / 2 : 1;  // binary64 groups need twice the registers
     if constexpr (!MIXED) {
-        spmv_class<0, U, false>(A, cr[0], cr[4], row0, x64, x32, q, qdummy);
+        spmv_class<0, U0, false>(A, cr[0], cr[4], row0, x64, x32, q, qdummy);
     } else {
-        spmv_class<0, U, DYN>(A, cr[0], cr[1], row0, x64, x32, q, qdummy);
+        spmv_class<0, U0, DYN>(A, cr[0], cr[1], row0, x64, x32, q, qdummy);
         spmv_class<1, U, DYN>(A, cr[1], cr[2], row0, x64, x32, q, qdummy);
         spmv_class<2, U, DYN>(A, cr[2], cr[3], row0, x64, x32, q, qdummy);
         spmv_class<3, U, DYN>(A, cr[3], cr[4], row0, x64, x32, q, qdummy);

@@ -4,11 +4,12 @@
 //
 // Device layout (HBM), derived from the Tile_create order:
 //   chunks  int3[C]    16-entry pieces of tiles {e0, column tile | class << 28,
-//                      tile row | (cnt-1) << 27}: entries [e0, e0+cnt)
-//   eidx    u8[nnz]    (local row << 4) | local column, in tile order
-//   v64     f64[nnz]   all values; v32/v16/v8 hold every value at that precision (the
+//                      tile row | (cnt-1) << 27}: positions [e0, e0+cnt), e0 % 4 == 0
+//   eidx    u8[P]      (local row << 4) | local column per position, tile order; the
+//                      positions between chunks (alignment) and the last 16 are zero
+//   v64     f64[P]     all values; v32/v16/v8 hold every value at that precision (the
 //                      tile's own class, or the copies read under dynamic classes)
-//   src     i32[nnz]   CSR position of each entry
+//   src     i32[P]     CSR position of each entry (-1 on padding)
 // A tile row's chunks are contiguous, so a workgroup owning a contiguous range of
 // tile rows owns a contiguous range of chunks.
 #include <hip/hip_runtime.h>
@@ -88,18 +89,26 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     L.tilem = ts.tilem;
     L.nnz = ts.nnz;
     L.tiles = T;
-    // chunks per tile row
+    // chunks and entry positions per tile row: a chunk's entries start at a multiple of
+    // 4 positions, so each lane of the SpMV loads 4 entries with one aligned load
     L.tr_chunk.assign(ts.tilem + 1, 0);
+    std::vector<int64_t> tr_pos(ts.tilem + 1, 0);
     for (int bi = 0; bi < ts.tilem; bi++) {
-        int64_t c = 0;
-        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
-            c += (ts.tile_off[t + 1] - ts.tile_off[t] + 15) / 16;
+        int64_t c = 0, p = 0;
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++) {
+            const int64_t m = ts.tile_off[t + 1] - ts.tile_off[t];
+            c += (m + 15) / 16;
+            p += (m / 16) * 16 + ((m % 16) + 3) / 4 * 4;
+        }
         L.tr_chunk[bi + 1] = L.tr_chunk[bi] + c;
+        tr_pos[bi + 1] = tr_pos[bi] + p;
     }
     const int64_t C = L.tr_chunk[ts.tilem];
     L.nchunks = C;
-    // 16 zero entries of padding, so a chunk's 16 lanes may load unconditionally
-    const int64_t P = ts.nnz + 16;
+    // 16 zero positions of padding at the end, so every lane may load unconditionally
+    const int64_t P = tr_pos[ts.tilem] + 16;
+    if (P >= INT32_MAX) return MF_ERR_TOO_LARGE;
+    L.npos = P;
     std::vector<int3> &ch = L.h_chunks;
     ch.assign(C > 0 ? C : 1, make_int3(0, 0, -1));
     std::vector<uint8_t> eid(P, 0);
@@ -122,33 +131,36 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     }
 #pragma omp parallel for schedule(dynamic, 64)
     for (int bi = 0; bi < ts.tilem; bi++) {
-        int64_t c = L.tr_chunk[bi];
+        int64_t c = L.tr_chunk[bi], pos = tr_pos[bi];
         for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++) {
             const int64_t a = ts.tile_off[t], b = ts.tile_off[t + 1];
             for (int64_t e0 = a; e0 < b; e0 += 16, c++) {
                 const int cnt = (int)std::min<int64_t>(16, b - e0);
-                ch[c] = make_int3((int)e0, (int)((unsigned)ts.tile_col[t] | ((unsigned)cls[t] << 28)),
+                ch[c] = make_int3((int)pos, (int)((unsigned)ts.tile_col[t] | ((unsigned)cls[t] << 28)),
                                   (int)((unsigned)bi | ((unsigned)(cnt - 1) << 27)));
-            }
-            for (int64_t e = a; e < b; e++) {
-                eid[e] = (uint8_t)((ts.lrow[e] << 4) | ts.lcol[e]);
-                const double v = val[ts.src[e]];
-                v64[e] = v;
-                src32[e] = (int)ts.src[e];
-                if (!v32.empty()) {
-                    // the tile's own class holds its rounding of the value; the lower
-                    // precisions hold the copies of main-cg-mixed.cu:2333-2350 (f32 RNE,
-                    // f16 of the f32, i8 truncated), read only under dynamic classes
-                    const float f = (float)v;
-                    v32[e] = f;
-                    v16[e] = cls[t] == MF_FP16 ? f64_to_f16_rne(v) : f32_to_f16_rne(f);
-                    if (cls[t] == MF_INT8) {
-                        v8[e] = (int8_t)v;
-                    } else {
-                        const double tv = std::trunc(v);
-                        v8[e] = std::isnan(v) ? 0 : (int8_t)(tv > 127.0 ? 127 : (tv < -128.0 ? -128 : tv));
+                for (int j = 0; j < cnt; j++) {
+                    const int64_t e = e0 + j, q = pos + j;
+                    eid[q] = (uint8_t)((ts.lrow[e] << 4) | ts.lcol[e]);
+                    const double v = val[ts.src[e]];
+                    v64[q] = v;
+                    src32[q] = (int)ts.src[e];
+                    if (!v32.empty()) {
+                        // the tile's own class holds its rounding of the value; the lower
+                        // precisions hold the copies of main-cg-mixed.cu:2333-2350 (f32
+                        // RNE, f16 of the f32, i8 truncated), read only under dynamic
+                        // classes
+                        const float f = (float)v;
+                        v32[q] = f;
+                        v16[q] = cls[t] == MF_FP16 ? f64_to_f16_rne(v) : f32_to_f16_rne(f);
+                        if (cls[t] == MF_INT8) {
+                            v8[q] = (int8_t)v;
+                        } else {
+                            const double tv = std::trunc(v);
+                            v8[q] = std::isnan(v) ? 0 : (int8_t)(tv > 127.0 ? 127 : (tv < -128.0 ? -128 : tv));
+                        }
                     }
                 }
+                pos += (cnt + 3) / 4 * 4;
             }
         }
     }
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(1024) void spmv_kernel(DevMatrix A, const double *_
     const int row0 = tr0 * kTile, row1 = min(tr1 * kTile, A.n);
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) q[i] = 0.0;
     __syncthreads();
-    spmv_chunks<MIXED, 4>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
+    spmv_chunks<MIXED, 2>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
     __syncthreads();
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
 }

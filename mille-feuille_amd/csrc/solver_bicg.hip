@@ -70,7 +70,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
     if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
     __syncthreads();
-    spmv_chunks<false, spmv_unroll<RPT>()>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
+    spmv_chunks<false, spmv_unroll<RPT, 4>()>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
     __syncthreads();
     double part[2] = {0.0, 0.0};
     R.refresh();
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
     R.refresh();
     const long long *cr = a.A.wg_chunk + 4 * (size_t)wg;
-    constexpr int U = spmv_unroll<RPT>();
+    constexpr int U = spmv_unroll<RPT, 4>();
     unsigned epoch = 0;
 
     double r[RPT], p[RPT], v[RPT], sv[RPT];

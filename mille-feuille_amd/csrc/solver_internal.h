@@ -28,7 +28,7 @@ void free_partition(Partition &P);
 
 struct DeviceLayout {
     int n_rows = 0, n_cols = 0, tilem = 0;
-    int64_t nnz = 0, tiles = 0, nchunks = 0;
+    int64_t nnz = 0, tiles = 0, nchunks = 0, npos = 0;
     std::vector<int64_t> tr_chunk;  // host, tilem+1
     std::vector<int3> h_chunks;     // host copy of the headers in tile order
     int3 *chunks = nullptr;

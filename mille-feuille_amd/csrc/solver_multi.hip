@@ -31,7 +31,7 @@ __global__ __launch_bounds__(1024) void mk_spmv(DevMatrix A, const double *__res
     const int row0 = tr0 * kTile, row1 = min(tr1 * kTile, A.n);
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) q[i] = 0.0;
     __syncthreads();
-    spmv_chunks<MIXED, 8>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
+    spmv_chunks<MIXED, 2>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
     __syncthreads();
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
 }
