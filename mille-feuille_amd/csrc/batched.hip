@@ -38,7 +38,7 @@ struct BatchArgs {
     const int *pool_sys0;       // npools+1 ranges into order
     unsigned *queue;            // npools counters
     int npools;
-    unsigned long long *recs;   // 16 u64 per workgroup
+    unsigned long long *recs;   // kRecWordsPerMember u64 per workgroup
     unsigned *ctl;
     long long timeout;
     const double *b;
@@ -65,7 +65,7 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
     __shared__ int s_idx;
     const int nsys = a.pool_sys0[pool + 1] - a.pool_sys0[pool];
     GridOps gsy;
-    gsy.gs.recs = a.recs + (size_t)g0 * 2 * kMaxRed * 2;
+    gsy.gs.recs = a.recs + (size_t)g0 * kRecWordsPerMember;  // disjoint per group
     gsy.gs.ctl = a.ctl;
     gsy.gs.timeout = a.timeout;
     gsy.gs.G = W;
@@ -446,7 +446,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o.max_replacements > 0 ? o.max_replacements : 64;
-    const long long nrec = 2LL * 2 * mfk::kMaxRed * B->G;
+    const long long nrec = (long long)mfk::kRecWordsPerMember * B->G;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MF_ERR_HIP;
     const long long key = ((long long)(mixed ? 1 : 0) << 40) ^ (long long)(intptr_t)d_b ^ ((long long)(intptr_t)d_x << 1) ^

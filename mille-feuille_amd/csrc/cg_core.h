@@ -25,6 +25,8 @@ struct CgArgs {
     int max_repl;
     uint8_t *dvis;
     int dyn_mode;
+    double *p64b;    // second publish buffer pair (pipelined variant)
+    float *p32b;
 };
 
 enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
@@ -67,6 +69,8 @@ struct CgPart {
     uint8_t *dvis;   // dynamic class per column tile (written by the owner of the tile row)
     int dyn_mode;    // 1 magnitude-relative, 2 the reference's |dp| thresholds
     int n_sys;       // rows of the system (for the RMS of p)
+    double *p64b;    // second publish buffer pair (pipelined variant)
+    float *p32b;
 };
 
 // This workgroup's share of x: in LDS (XREG = false) or in registers (XREG = true).
@@ -387,6 +391,236 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     return res;
 }
 
+// Publishes v (rows of this workgroup) into publish buffer `tb` (0: p64/p32, 1: the
+// second pair), binary64 and/or binary32 as the SpMV that reads it needs.
+template <int RPT, int BLK>
+__device__ __forceinline__ void publish_buf(const CgPart &a, Rows<RPT, BLK> &R, const double (&v)[RPT], int tb,
+                                            bool need64, bool need32)
+{
+    double *d64 = tb ? a.p64b : a.p64;
+    float *d32 = tb ? a.p32b : a.p32;
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        if (need64) d64[row] = v[k];
+        if (need32) d32[row] = (float)v[k];
+    }
+}
+
+// q <- A v for the vector published in buffer `tb` (tile classes when MIXED).
+template <int RPT, bool MIXED, int BLK, int U>
+__device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int tb, double *q, double *scratch)
+{
+    zero_q<RPT, BLK>(q, R);
+    __syncthreads();
+    spmv_chunks<MIXED, U>(a.A, a.cr, R.row0, tb ? a.p64b : a.p64, tb ? a.p32b : a.p32, q, scratch);
+    __syncthreads();
+}
+
+// Pipelined CG (Ghysels & Vanroose 2014, unpreconditioned): with w = A r, s = A p and
+// z = A s carried by recurrences, the two inner products of an iteration, (r, r) and
+// (w, r), need only local vectors, so their grid reduction is also the barrier that
+// publishes w for the iteration's single SpMV q = A w:
+//   gamma = (r,r), delta = (w,r)                      one grid reduction + publish w
+//   q = A w
+//   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+//   z = q + beta z, s = w + beta s, p = r + beta p, x += alpha p, r -= alpha s, w -= alpha z
+// One grid synchronisation per iteration instead of two.  Published vectors alternate
+// between two buffers: a workgroup can publish the next vector while another still
+// gathers the previous one, and cannot get two publishes ahead (the reduction between
+// them waits for every workgroup).  The recurrences drift from their definitions faster
+// than plain CG's, so every 3 decades of residual reduction (replace_delta; every decade
+// with binary32 products), and before accepting convergence, r = b - A x (binary64
+// stream), w = A r, s = A p and z = A s are recomputed from their definitions, keeping p.  Five row vectors live in registers,
+// so the launcher uses this variant only up to 2 rows per thread.
+template <int RPT, bool MIXED, int BLK, class Sync, int U, bool XREG>
+__device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
+                                      double *xs_lds)
+{
+    XVec<RPT, XREG> xs;
+    xs.lds = xs_lds;
+    CgResult res;
+    res.it = 0;
+    res.repl = 0;
+    res.status = ST_NOTCONV;
+    res.rr = res.bb = 0.0;
+    res.rt = -1.0;
+    const bool n64 = !MIXED || a.need64;
+    const bool n32 = MIXED && a.need32;
+    double r[RPT], w[RPT], p[RPT], sv[RPT], z[RPT];
+    int tb = 0;
+    // ---- setup: r = b - A x0 ----
+    if (a.use_x0) {
+        zero_q<RPT, BLK>(q, R);
+        __syncthreads();
+        spmv_chunks<false, U>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
+        __syncthreads();
+    }
+    double part[2] = {0.0, 0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        r[k] = w[k] = p[k] = sv[k] = z[k] = 0.0;
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        const double bk = a.b[row];
+        xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
+        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        part[0] += r[k] * r[k];
+        part[1] += bk * bk;
+    }
+    double bb = 0.0, rr = 0.0, rbase = 0.0, gamma_old = 1.0, alpha_old = 1.0;
+    bool first = true, fresh = false;
+    // with binary32 products the w and z recurrences amplify the SpMV rounding: replace
+    // at least every decade of residual reduction (every three stalls the recurrence)
+    const double delta2 = MIXED ? fmax(a.delta2, 1e-2) : a.delta2;
+    // w = A r
+    publish_buf<RPT, BLK>(a, R, r, tb, n64, n32);
+    if (!sy.template sum<2>(part, true, scratch)) {
+        res.status = ST_TIMEOUT;
+        goto finish;
+    }
+    rr = part[0];
+    bb = part[1];
+    rbase = rr;
+    if (bb == 0.0) {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) xs.set(k, R.loc(k), 0.0);
+        res.status = ST_OK;
+        rr = 0.0;
+        goto finish;
+    }
+    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+    tb ^= 1;
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+
+    for (;;) {
+        // gamma = (r, r), delta = (w, r); the same reduction publishes w
+        double gd[2] = {0.0, 0.0};
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            gd[0] += r[k] * r[k];
+            gd[1] += w[k] * r[k];
+        }
+        publish_buf<RPT, BLK>(a, R, w, tb, n64, n32);
+        if (!sy.template sum<2>(gd, true, scratch)) {
+            res.status = ST_TIMEOUT;
+            break;
+        }
+        const double gamma = gd[0], delta = gd[1];
+        rr = gamma;
+        if (!a.fixed) {
+            const bool conv = gamma <= a.tol2 * bb;
+            if (!fresh && res.repl < a.max_repl && (conv || gamma < delta2 * rbase)) {
+                // recompute r, w, s, z from their definitions (p kept), then redo the
+                // reduction with the fresh vectors
+                double rr_new = 0.0;
+                if (!replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr_new, false)) {
+                    res.status = ST_TIMEOUT;
+                    break;
+                }
+                double dummy[1] = {0.0};
+                publish_buf<RPT, BLK>(a, R, r, tb, n64, n32);
+                if (!sy.template sum<1>(dummy, true, scratch)) {
+                    res.status = ST_TIMEOUT;
+                    break;
+                }
+                spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                tb ^= 1;
+                R.refresh();
+#pragma unroll
+                for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                if (!first) {
+                    publish_buf<RPT, BLK>(a, R, p, tb, n64, n32);
+                    if (!sy.template sum<1>(dummy, true, scratch)) {
+                        res.status = ST_TIMEOUT;
+                        break;
+                    }
+                    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                    tb ^= 1;
+                    R.refresh();
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) sv[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                    publish_buf<RPT, BLK>(a, R, sv, tb, n64, n32);
+                    if (!sy.template sum<1>(dummy, true, scratch)) {
+                        res.status = ST_TIMEOUT;
+                        break;
+                    }
+                    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                    tb ^= 1;
+                    R.refresh();
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) z[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                }
+                res.repl++;
+                rbase = rr_new;
+                fresh = true;
+                continue;
+            }
+            if (conv) {
+                res.status = ST_OK;
+                break;
+            }
+        }
+        fresh = false;
+        if (res.it >= a.maxit) {
+            res.status = a.fixed ? ST_OK : ST_NOTCONV;
+            break;
+        }
+        // q = A w (w published by the reduction above)
+        spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+        tb ^= 1;
+        double beta = 0.0, alpha;
+        if (first) {
+            alpha = gamma / delta;
+        } else {
+            beta = gamma / gamma_old;
+            alpha = gamma / (delta - beta * gamma / alpha_old);
+        }
+        if (!isfinite(alpha) || alpha == 0.0 || !isfinite(beta)) {
+            res.status = ST_BREAK;
+            break;
+        }
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const double qk = R.ok(k) ? q[R.loc(k)] : 0.0;
+            z[k] = qk + beta * z[k];
+            sv[k] = w[k] + beta * sv[k];
+            p[k] = r[k] + beta * p[k];
+            if (R.ok(k)) xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
+            r[k] -= alpha * sv[k];
+            w[k] -= alpha * z[k];
+        }
+        gamma_old = gamma;
+        alpha_old = alpha;
+        first = false;
+        res.it++;
+    }
+finish:
+    // write x, then the binary64 true residual ||b - A x||^2 (r is dead from here on)
+    if (res.status != ST_TIMEOUT) {
+        double rr_t = 0.0;
+        if (replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr_t, false)) res.rt = rr_t;
+        else res.status = ST_TIMEOUT;
+    } else {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+    }
+    res.rr = rr;
+    res.bb = bb;
+    return res;
+}
+
 template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN>
 __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
 {
@@ -423,8 +657,15 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.dvis = a.dvis;
     part.dyn_mode = a.dyn_mode;
     part.n_sys = a.A.n;
-    CgResult res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT, (XREG || DYN) ? 3 : 2>(), XREG, DYN>(
-        part, R, sy, scratch, q, xs);
+    part.p64b = a.p64b;
+    part.p32b = a.p32b;
+    CgResult res;
+    if constexpr (VAR == 3)
+        res = cg_core_pipelined<RPT, MIXED, BLK, GridOps, spmv_unroll<RPT, XREG ? 6 : 5>(), XREG>(part, R, sy, scratch,
+                                                                                                 q, xs);
+    else
+        res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT, (XREG || DYN) ? 3 : 2>(), XREG, DYN>(
+            part, R, sy, scratch, q, xs);
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)res.it;
         a.info[1] = res.rr;

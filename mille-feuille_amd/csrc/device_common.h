@@ -39,7 +39,7 @@ __device__ __forceinline__ int chunk_last(int3 h) { return ((unsigned)h.z >> 27)
 
 // Flagged-record grid barrier state.
 struct GridSync {
-    unsigned long long *recs;   // [2][kMaxRed][G] records of 2 x u64 {lo|epoch, hi|epoch}
+    unsigned long long *recs;   // [2][kMaxRed][G + 1] records of 2 x u64 {lo|epoch, hi|epoch}
     unsigned *ctl;              // ctl[0] abort flag
     long long timeout;          // cycles of s_memtime a wait may last
     int G;
@@ -92,66 +92,121 @@ __device__ __forceinline__ void put_record(unsigned long long *rec, double v, un
     __hip_atomic_store(rec + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide deterministic all-reduce of K doubles with a barrier.
+// Records of one synchronisation group of G workgroups: for each epoch parity and
+// reduced value k, G member records (index = member) and one broadcast record (index G),
+// each 2 x u64 {lo32 | epoch << 32, hi32 | epoch << 32}.  A group needs rec_words(G)
+// u64; kRecWordsPerMember * G always suffices.
+constexpr int kRecWordsPerMember = 4 * kMaxRed * 2;
+__host__ __device__ constexpr long long rec_words(int G) { return 2LL * kMaxRed * (G + 1) * 2; }
+
+__device__ __forceinline__ unsigned long long *rec_at(const GridSync &gs, unsigned epoch, int k, int i)
+{
+    return gs.recs + (((size_t)(epoch & 1) * kMaxRed + k) * (size_t)(gs.G + 1) + i) * 2;
+}
+
+constexpr unsigned kAbortTag = 0x80000000u;  // broadcast tag bit: the group aborts at this sync
+
+// Waits for the record to carry `epoch` (or, with accept_abort, epoch | kAbortTag, which
+// sets *aborted).  False when the watchdog fired or the abort flag is set; then the abort
+// flag is set for everyone.
+__device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned long long *rec, unsigned epoch,
+                                            long long t0, double &out, bool accept_abort = false,
+                                            bool *aborted = nullptr)
+{
+    for (;;) {
+        const unsigned long long lo = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long hi = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned tl = (unsigned)(lo >> 32), th = (unsigned)(hi >> 32);
+        if (tl == th && (tl == epoch || (accept_abort && tl == (epoch | kAbortTag)))) {
+            out = __longlong_as_double((long long)((lo & 0xffffffffull) | (hi << 32)));
+            if (aborted) *aborted = tl != epoch;
+            return true;
+        }
+        if (__hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+            __builtin_readcyclecounter() - t0 > gs.timeout) {
+            __hip_atomic_store(gs.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Grid-wide deterministic all-reduce of K doubles with a barrier (gather to member 0,
+// broadcast back).
 //   v[k]        : this thread's partial; on return the grid total (same bits in every
-//                 workgroup, fixed summation order).
+//                 workgroup: member 0 sums the member records in a fixed order and every
+//                 member takes its result).
 //   visibility  : also make every global store issued before the call visible to all
-//                 workgroups after it (release before the records, acquire after).
-// Returns false when the watchdog fired or another workgroup aborted.
+//                 workgroups after it (release before the member record, acquire and
+//                 release at member 0 between gather and broadcast, acquire after).
+// Every member publishes one record per value; member 0 polls all of them, the others
+// poll only the broadcast record with K lanes.  Polling every member record from every
+// workgroup put hundreds of thousands of agent-scope loads on a few cache lines and
+// cost about 10 us per reduction at 256 workgroups; two hops with few pollers cost far
+// less.  A failed wait (watchdog, or the abort flag set by another workgroup) travels in
+// the broadcast tag, so every member leaves at the same sync without a further read of
+// the flag.  Returns false when the group aborts.
 template <int K>
 __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bool visibility, int wg,
                          double *scratch)
 {
+    __shared__ double s_bc[kMaxRed];
+    __shared__ int s_fail;
     if (visibility) wait_own_stores();
+    if (threadIdx.x == 0) s_fail = 0;
     block_sum<K>(v, scratch);
     epoch++;
     const int G = gs.G;
-    unsigned long long *buf = gs.recs + (size_t)(epoch & 1) * kMaxRed * G * 2;
-    if (threadIdx.x < 64) {  // wave 0 publishes
-        if (visibility) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (threadIdx.x < K) put_record(buf + ((size_t)threadIdx.x * G + wg) * 2, v[threadIdx.x], epoch);
-    }
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) acc[k] = 0.0;
-    bool ok = true;
     const long long t0 = __builtin_readcyclecounter();
-    for (int i = threadIdx.x; i < K * G; i += blockDim.x) {
-        const unsigned long long *rec = buf + (size_t)i * 2;
-        unsigned long long lo, hi;
-        for (;;) {
-            lo = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            hi = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((unsigned)(lo >> 32) == epoch && (unsigned)(hi >> 32) == epoch) break;
-            if (__hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                __builtin_readcyclecounter() - t0 > gs.timeout) {
-                __hip_atomic_store(gs.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = false;
+    if (threadIdx.x < 64) {  // wave 0 publishes this member's record
+        if (visibility) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (threadIdx.x < K) put_record(rec_at(gs, epoch, threadIdx.x, wg), v[threadIdx.x], epoch);
+    }
+    // keep the compiler from moving the record stores behind this thread's polling
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (wg == 0) {
+        double acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = 0.0;
+        for (int i = threadIdx.x; i < K * G; i += blockDim.x) {
+            const int k = i / G;
+            double d;
+            if (!poll_record(gs, rec_at(gs, epoch, k, i - k * G), epoch, t0, d)) {
+                s_fail = 1;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int kk = 0; kk < K; kk++)
+                if (kk == k) acc[kk] += d;
         }
-        if (!ok) break;
-        double d = __longlong_as_double((long long)((lo & 0xffffffffull) | (hi << 32)));
-        const int k = i / G;
+        block_sum<K>(acc, scratch);  // fixed order: per thread, then waves, then block
+        const bool fail = s_fail != 0;
+        if (threadIdx.x < 64) {
+            if (visibility) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            }
+            if (threadIdx.x < K)
+                put_record(rec_at(gs, epoch, threadIdx.x, G), acc[threadIdx.x], fail ? (epoch | kAbortTag) : epoch);
+        }
 #pragma unroll
-        for (int kk = 0; kk < K; kk++)
-            if (kk == k) acc[kk] += d;
+        for (int k = 0; k < K; k++) v[k] = acc[k];
+        __syncthreads();  // s_fail is reset by the next call
+        return !fail;
     }
-    block_sum<K>(acc, scratch);  // every thread's polls are done after its first __syncthreads
-    if (visibility) {
-        if (threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __syncthreads();
+    if (threadIdx.x < K) {
+        double d = 0.0;
+        bool ab = false;
+        if (!poll_record(gs, rec_at(gs, epoch, threadIdx.x, G), epoch, t0, d, true, &ab) || ab) s_fail = 1;
+        s_bc[threadIdx.x] = d;
     }
-    // abort is sticky: re-read the flag so every workgroup leaves at the same sync
-    __shared__ int s_ab;
-    if (threadIdx.x == 0) s_ab = __hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-    __syncthreads();
-    ok = !s_ab;
+    if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] = acc[k];
-    return ok;
+    for (int k = 0; k < K; k++) v[k] = s_bc[k];
+    const bool fail = s_fail != 0;
+    __syncthreads();  // s_bc and s_fail are rewritten by the next call
+    return !fail;
 }
 
 __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long e)

@@ -59,7 +59,7 @@ mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, B, lds) != hipSuccess) return MF_ERR_HIP;
     if (G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
-    const long long nrec = 2LL * 2 * mfk::kMaxRed * G;
+    const long long nrec = mfk::rec_words(G);
     hipLaunchKernelGGL(mfk::reset_sync_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl, A->ws.info);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return MF_ERR_HIP;

@@ -306,7 +306,7 @@ mf_status Workspace::ensure(int64_t n, int G, hipStream_t s)
     }
     if (G > cap_g) {
         free_dev(recs);
-        if (hipMalloc((void **)&recs, sizeof(unsigned long long) * 2 * 2 * mfk::kMaxRed * (size_t)G) != hipSuccess)
+        if (hipMalloc((void **)&recs, sizeof(unsigned long long) * mfk::kRecWordsPerMember * (size_t)G) != hipSuccess)
             return MF_ERR_ALLOC;
         cap_g = G;
     }

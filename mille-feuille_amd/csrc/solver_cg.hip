@@ -17,6 +17,9 @@
 //            the first sync (r'q and q'q reduced together with p'q) for beta, and the
 //            directly summed r'r rides on the publish sync for the next alpha and the
 //            convergence test.
+// variant 3: one grid sync (pipelined CG, see cg_core_pipelined): w = A r, s = A p and
+//            z = A s by recurrence, (r,r) and (w,r) reduced by the sync that publishes
+//            w for the iteration's SpMV.
 // Mixed precision: the SpMV reads every tile at its storage class with binary32
 // products (p converted to binary32 by its owner when published); x, r, the dot
 // products and scalars stay binary64.  The residual is replaced by b - A x computed
@@ -66,7 +69,20 @@ static CgKernel pick_cg(int rpt)
     }
 }
 
-// Dynamic classes are instantiated for the mixed, two-sync variant only.
+// Pipelined variant: five register vectors, instantiated up to 2 rows per thread
+// (more would spill).
+template <bool MIXED, bool XREG>
+static CgKernel pick_cg3(int rpt)
+{
+    switch (rpt) {
+    case 1: return mfk::cg_kernel<1, 3, MIXED, 1024, XREG, false>;
+    case 2: return mfk::cg_kernel<2, 3, MIXED, 1024, XREG, false>;
+    default: return nullptr;
+    }
+}
+
+// Variants: 1 textbook (3 syncs per iteration), 2 fused reductions (2 syncs),
+// 3 pipelined (1 sync).  Dynamic classes are instantiated for variant 2 only.
 static CgKernel cg_kernel_for(int rpt, int var, bool mixed, bool xreg, bool dyn)
 {
     if (dyn) {
@@ -75,9 +91,11 @@ static CgKernel cg_kernel_for(int rpt, int var, bool mixed, bool xreg, bool dyn)
     }
     if (xreg) {
         if (var == 1) return mixed ? pick_cg<1, true, true, false>(rpt) : pick_cg<1, false, true, false>(rpt);
+        if (var == 3) return mixed ? pick_cg3<true, true>(rpt) : pick_cg3<false, true>(rpt);
         return mixed ? pick_cg<2, true, true, false>(rpt) : pick_cg<2, false, true, false>(rpt);
     }
     if (var == 1) return mixed ? pick_cg<1, true, false, false>(rpt) : pick_cg<1, false, false, false>(rpt);
+    if (var == 3) return mixed ? pick_cg3<true, false>(rpt) : pick_cg3<false, false>(rpt);
     return mixed ? pick_cg<2, true, false, false>(rpt) : pick_cg<2, false, false, false>(rpt);
 }
 
@@ -114,7 +132,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     const bool dyn = o->precision == MF_PREC_MIXED && o->dynamic_precision != 0 && A->L.has_copies;
     const bool mixed = o->precision == MF_PREC_MIXED && (A->L.has_lowp || dyn);
     const int tv = A->tuned_variant[mixed ? 1 : 0];
-    const int var = dyn ? 2 : (o->variant == 1 ? 1 : (o->variant == 2 ? 2 : (tv == 1 ? 1 : 2)));
+    const int var = dyn ? 2 : (o->variant >= 1 && o->variant <= 3 ? o->variant : (tv >= 1 && tv <= 3 ? tv : 2));
     if (A->L.tilem == 0) return MF_ERR_INVALID;
     const int G = choose_grid(A, o, ((long long)var << 8) | (mixed ? 1 : 0));
     if (G > A->L.tilem) return MF_ERR_INVALID;
@@ -132,7 +150,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     }
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
     CgKernel k = cg_kernel_for(rpt, var, mixed, xreg, dyn);
-    if (!k) return MF_ERR_TOO_LARGE;
+    if (!k) return var == 3 ? MF_ERR_UNSUPPORTED : MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
 
     mfk::CgArgs a;
@@ -145,6 +163,8 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.x = d_x;
     a.p64 = A->ws.p64;
     a.p32 = A->ws.p32;
+    a.p64b = A->ws.s64;
+    a.p32b = A->ws.s32;
     a.info = A->ws.info;
     a.tol2 = o->tol * o->tol;
     const double dl = o->replace_delta > 0 ? o->replace_delta : 1e-3;
@@ -228,7 +248,7 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
     const int cands[] = {8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 512};
     double best = 1e30;
     int bg = 0, bv = 2;
-    for (int var = 1; var <= 2; var++) {
+    for (int var = 1; var <= 3; var++) {
         for (int G : cands) {
             if (G > A->L.tilem || G > 2 * A->num_cus) continue;
             mf_solve_options o;
@@ -254,7 +274,7 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
         }
     }
     if (bg == 0) return MF_ERR_TOO_LARGE;
-    for (int var = 1; var <= 2; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
+    for (int var = 1; var <= 3; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
     A->tuned_variant[mixed ? 1 : 0] = bv;
     if (best_grid) *best_grid = bg;
     if (best_variant) *best_variant = bv;

@@ -44,6 +44,14 @@ def orc():
     return oracle
 
 
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
 TILE_CASES = ["add20", "poisson64", "ragged1000", "ragged1009", "densetile", "emptyrows", "unsorted"]
 
 

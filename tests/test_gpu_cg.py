@@ -101,7 +101,7 @@ def test_spmv_mixed_matches_oracle_emulation(mf, orc, torch, name):
     A.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("name", ["poisson64", "poisson2d_ragged", "poisson2d_one_row_tail", "shifted_random_spd"])
 def test_cg_fp64_iterations_and_residual(mf, orc, torch, name, variant):
     rp, ci, va, b, xs = _case(mf, name)
@@ -130,7 +130,7 @@ def test_cg_grid_sizes_agree(mf, orc, torch, grid):
     A.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("name", ["poisson64", "shifted_random_spd"])
 def test_cg_mixed_reaches_fp64_tolerance(mf, orc, torch, name, variant):
     rp, ci, va, b, xs = _case(mf, name)
@@ -156,6 +156,28 @@ def test_cg_zero_rhs_and_max_iter(mf, orc, torch):
     assert info["status"] == 0 and info["iterations"] == 7
     ref = orc.cg(rp, ci, va, b, tol=0.0, maxit=7)
     assert np.allclose(x.cpu().numpy(), ref["x"], rtol=1e-9, atol=1e-12)
+    A.close()
+
+
+def test_cg_pipelined_edge_cases(mf, orc, torch):
+    """Variant 3 (one grid sync per iteration): zero right-hand side, a fixed iteration
+    budget against the textbook recurrence, an initial guess, and its refusal beyond the
+    rows per thread it is built for."""
+    rp, ci, va, b, xs = _case(mf, "poisson64")
+    A = mf.TiledMatrix(rp, ci, va)
+    x, info = A.cg(np.zeros_like(b), variant=3)
+    assert info["status"] == 0 and info["iterations"] == 0 and float(x.abs().max()) == 0.0
+    x, info = A.cg(b, max_iter=7, fixed_iterations=True, variant=3)
+    assert info["status"] == 0 and info["iterations"] == 7
+    ref = orc.cg(rp, ci, va, b, tol=0.0, maxit=7)
+    assert np.allclose(x.cpu().numpy(), ref["x"], rtol=1e-9, atol=1e-12)
+    x0 = xs + 1e-3 * np.random.default_rng(1).standard_normal(xs.size)
+    x, info = A.cg(b, x0=x0, tol=1e-10, variant=3)
+    assert info["status"] == 0
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    # one workgroup holding all 4096 rows needs 4 rows per thread: not built for variant 3
+    with pytest.raises(mf.MfError):
+        A.cg(b, grid=1, variant=3)
     A.close()
 
 
