@@ -103,8 +103,8 @@ __device__ __forceinline__ void publish_p(const CgPart &a, Rows<RPT, BLK> &R, co
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
-        if (a.need64) a.p64[row] = p[k];
-        if (a.need32) a.p32[row] = (float)p[k];
+        if (a.need64) store_wt(&a.p64[row], p[k]);
+        if (a.need32) store_wt(&a.p32[row], (float)p[k]);
     }
 }
 
@@ -118,7 +118,7 @@ __device__ __forceinline__ void classify_segments(const CgPart &a, Rows<RPT, BLK
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int c = dyn_segment_class(a.dyn_mode, p_old[k], p_new[k], rms);
-        if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = (uint8_t)c;
+        if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) store_wt(&a.dvis[(R.row0 + R.loc(k)) >> 4], (uint8_t)c);
     }
 }
 
@@ -131,7 +131,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+        if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs.get(k, R.loc(k)));
     double dummy[1] = {0.0};
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
@@ -196,7 +196,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = 0;
+            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) store_wt(&a.dvis[(R.row0 + R.loc(k)) >> 4], (uint8_t)0);
     }
     double rr = 0.0, bb = 0.0, rbase = 0.0, pp = 0.0;
     bool dyn_on = DYN;
@@ -251,7 +251,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                         R.refresh();
 #pragma unroll
                         for (int k = 0; k < RPT; k++)
-                            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) a.dvis[(R.row0 + R.loc(k)) >> 4] = 0;
+                            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows) store_wt(&a.dvis[(R.row0 + R.loc(k)) >> 4], (uint8_t)0);
                         double z[1] = {0.0};
                         if (!sy.template sum<1>(z, true, scratch)) {
                             res.status = ST_TIMEOUT;
@@ -404,8 +404,8 @@ __device__ __forceinline__ void publish_buf(const CgPart &a, Rows<RPT, BLK> &R, 
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
-        if (need64) d64[row] = v[k];
-        if (need32) d32[row] = (float)v[k];
+        if (need64) store_wt(&d64[row], v[k]);
+        if (need32) store_wt(&d32[row], (float)v[k]);
     }
 }
 

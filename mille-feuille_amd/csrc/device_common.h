@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace mfk {
 
@@ -56,11 +57,41 @@ __device__ __forceinline__ int logical_wg(int b, int G)
     return (b % 8) * (G / 8) + b / 8;
 }
 
+// v moved across lanes by a DPP pattern (CTRL), both halves of the double.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wave, the same bits in every wave for the same inputs:
+// rotations within each 16-lane row (DPP row_ror 8, 4, 2, 1) give every lane its row
+// sum, then the four row sums are combined in a fixed order.  All lanes must be active.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_move<0x128>(v);  // row_ror:8
+    v += dpp_move<0x124>(v);  // row_ror:4
+    v += dpp_move<0x122>(v);  // row_ror:2
+    v += dpp_move<0x121>(v);  // row_ror:1
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Store that writes through to the agent coherence point (vector store with sc1), for
+// data other workgroups read after the next grid synchronisation.  Every such store
+// is complete once the storing wave has waited for its stores (wait_own_stores), so the
+// synchronisation needs no L2 write-back.
+template <class T>
+__device__ __forceinline__ void store_wt(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Deterministic sum of K per-thread values over the block; result in every thread.
@@ -136,9 +167,10 @@ __device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned l
 //   v[k]        : this thread's partial; on return the grid total (same bits in every
 //                 workgroup: member 0 sums the member records in a fixed order and every
 //                 member takes its result).
-//   visibility  : also make every global store issued before the call visible to all
-//                 workgroups after it (release before the member record, acquire and
-//                 release at member 0 between gather and broadcast, acquire after).
+//   visibility  : also make the data every member stored with store_wt before the call
+//                 visible to all members after it: each wave waits for its write-through
+//                 stores before the record is published (no L2 write-back is needed), and
+//                 every member invalidates its L2 (agent acquire) once it has the result.
 // Every member publishes one record per value; member 0 polls all of them, the others
 // poll only the broadcast record with K lanes.  Polling every member record from every
 // workgroup put hundreds of thousands of agent-scope loads on a few cache lines and
@@ -158,10 +190,9 @@ __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bo
     epoch++;
     const int G = gs.G;
     const long long t0 = __builtin_readcyclecounter();
-    if (threadIdx.x < 64) {  // wave 0 publishes this member's record
-        if (visibility) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (threadIdx.x < K) put_record(rec_at(gs, epoch, threadIdx.x, wg), v[threadIdx.x], epoch);
-    }
+    // wave 0 publishes this member's record; with visibility every wave has waited for
+    // its write-through stores before the barrier in block_sum, so they are complete
+    if (threadIdx.x < K) put_record(rec_at(gs, epoch, threadIdx.x, wg), v[threadIdx.x], epoch);
     // keep the compiler from moving the record stores behind this thread's polling
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (wg == 0) {
@@ -181,14 +212,10 @@ __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bo
         }
         block_sum<K>(acc, scratch);  // fixed order: per thread, then waves, then block
         const bool fail = s_fail != 0;
-        if (threadIdx.x < 64) {
-            if (visibility) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            }
-            if (threadIdx.x < K)
-                put_record(rec_at(gs, epoch, threadIdx.x, G), acc[threadIdx.x], fail ? (epoch | kAbortTag) : epoch);
-        }
+        if (threadIdx.x < K)
+            put_record(rec_at(gs, epoch, threadIdx.x, G), acc[threadIdx.x], fail ? (epoch | kAbortTag) : epoch);
+        // after the broadcast (off the other members' critical path): drop stale L2 lines
+        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = acc[k];
         __syncthreads();  // s_fail is reset by the next call
@@ -217,11 +244,12 @@ __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long
 }
 
 // Chunk groups (16 chunks per wave each) per batch of the SpMV for a kernel holding VEC
-// binary64 vectors of RPT rows per thread in registers.  Two groups keep about 80 KB per
-// CU in flight across the three pipeline stages, well above what covers HBM latency;
-// one group when the row vectors already take 48+ registers, so nothing spills.
+// binary64 vectors of RPT rows per thread in registers.  With the four-stage pipeline,
+// one group per batch keeps about 34 KB per CU in flight, above what covers HBM latency;
+// two groups when the row vectors leave room (fewer than 32 of their registers), so
+// nothing spills.
 template <int RPT, int VEC = 2>
-constexpr int spmv_unroll() { return RPT * VEC >= 24 ? 1 : 2; }
+constexpr int spmv_unroll() { return RPT * VEC >= 16 ? 1 : 2; }
 
 // Four consecutive values starting at a multiple-of-4 position, loaded with one
 // aligned access (two for binary64), and value j of such a group.
@@ -260,13 +288,17 @@ __device__ __forceinline__ auto quad_value(const Q &v, int j)
 }
 
 // Pipeline registers of one batch of the SpMV: chunk headers, entry indices (4 per
-// lane) and values (4 per lane), and whether each chunk exists.
+// lane), values (4 per lane), the gathered x of those 4 entries, and whether each chunk
+// exists (and, under dynamic classes, its column tile's dynamic class).
 template <int CLS, int U>
 struct SpmvStage {
     using QT = decltype(load_quad<CLS>(DevMatrix{}, 0));
+    using XT = std::conditional_t<CLS == 0, double, float>;
     int3 h[U];
     unsigned id[U];
     QT val[U];
+    XT xg[U][4];
+    int dc[U];
     bool v[U];
 };
 
@@ -292,70 +324,76 @@ __device__ __forceinline__ void spmv_load_entries(const DevMatrix &A, SpmvStage<
     }
 }
 
-// One pipeline step: X holds batch j (entries loaded), Y batch j+1 (headers loaded).
-// Gathers and accumulates batch j, loads Y's entries, and reloads X's headers with
-// batch j+2 (cb2 = its first chunk).
+// x of the 4 entries of every chunk.  Inactive entries read x at the chunk's first
+// column: in range (the tile exists), inside the line the active lanes read anyway, and
+// never a single address shared by every padded entry of the grid (which concentrated
+// the requests of all CUs on one L2 channel).
 template <int CLS, int U, bool DYN>
-__device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> &X, SpmvStage<CLS, U> &Y,
-                                          int cb2, int last, int row0, int sub, int lane,
-                                          const double *__restrict__ x64, const float *__restrict__ x32,
-                                          double *q, double *qdummy)
+__device__ __forceinline__ void spmv_load_gathers(const DevMatrix &A, SpmvStage<CLS, U> &S, int sub,
+                                                  const double *__restrict__ x64, const float *__restrict__ x32)
 {
-    double xg64[U][4];
-    float xg32[U][4];
-    int qi[U][4], dcls[U];
-    long long ebase[U];
-    (void)xg64;
-    (void)ebase;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const int lastk = X.v[u] ? chunk_last(X.h[u]) : -1;
-        const int cbase = chunk_colbase(X.h[u]), rbase = chunk_rowbase(X.h[u]) - row0;
-        dcls[u] = DYN ? (int)A.dvis[chunk_coltile(X.h[u])] : 0;
-        ebase[u] = (long long)X.h[u].x + 4 * sub;
+        const int lastk = S.v[u] ? chunk_last(S.h[u]) : -1;
+        const int cbase = chunk_colbase(S.h[u]);
+        S.dc[u] = DYN ? (int)A.dvis[chunk_coltile(S.h[u])] : 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const bool act = 4 * sub + j <= lastk;
-            const int id = (X.id[u] >> (8 * j)) & 255;
-            const int col = act ? cbase + (id & 15) : 0;
-            qi[u][j] = act ? rbase + (id >> 4) : -1;
-            if constexpr (CLS == 0) xg64[u][j] = x64[col];
-            if constexpr (CLS != 0 || DYN) xg32[u][j] = x32[col];
+            const int col = cbase + (4 * sub + j <= lastk ? (int)((S.id[u] >> (8 * j)) & 15) : 0);
+            if constexpr (CLS == 0) S.xg[u][j] = x64[col];
+            else S.xg[u][j] = x32[col];
         }
     }
-    spmv_load_entries<CLS, U>(A, Y, sub);
-    double prod[U][4];
+}
+
+// Products of a batch whose values and gathers have arrived, added into q (inactive
+// entries add 0.0 to a lane-private dummy word; the mask is a multiplication so the
+// product and its loads are never moved into a branch).
+template <int CLS, int U, bool DYN>
+__device__ __forceinline__ void spmv_accumulate(const DevMatrix &A, const SpmvStage<CLS, U> &S, int row0, int sub,
+                                                int lane, const float *__restrict__ x32, double *q, double *qdummy)
+{
 #pragma unroll
-    for (int u = 0; u < U; u++)
+    for (int u = 0; u < U; u++) {
+        const int lastk = S.v[u] ? chunk_last(S.h[u]) : -1;
+        const int rbase = chunk_rowbase(S.h[u]) - row0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            if constexpr (CLS == 0) prod[u][j] = quad_value<CLS>(X.val[u], j) * xg64[u][j];
-            else prod[u][j] = (double)(quad_value<CLS>(X.val[u], j) * xg32[u][j]);
-        }
-    if constexpr (DYN) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (dcls[u] > CLS) {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (qi[u][j] < 0) continue;
+            bool act = 4 * sub + j <= lastk;
+            const int qi = rbase + ((S.id[u] >> (8 * j + 4)) & 15);
+            double prod;
+            if constexpr (CLS == 0) prod = quad_value<CLS>(S.val[u], j) * S.xg[u][j];
+            else prod = (double)(quad_value<CLS>(S.val[u], j) * S.xg[u][j]);
+            if constexpr (DYN) {
+                if (act && S.dc[u] > CLS) {
                     // lowered below the storage class: the value copy at that precision
-                    if (dcls[u] >= 4) qi[u][j] = -1;
-                    else prod[u][j] = (double)(val_lowp(A, dcls[u], ebase[u] + j) * xg32[u][j]);
+                    if (S.dc[u] >= 4) {
+                        act = false;
+                    } else {
+                        const int col = chunk_colbase(S.h[u]) + ((S.id[u] >> (8 * j)) & 15);
+                        prod = (double)(val_lowp(A, S.dc[u], (long long)S.h[u].x + 4 * sub + j) * x32[col]);
+                    }
                 }
             }
+            atomicAdd(act ? &q[qi] : &qdummy[lane], prod * (act ? 1.0 : 0.0));
         }
     }
-    spmv_load_headers<CLS, U>(A, X, cb2, last);
-    // inactive entries add 0.0 to a lane-private dummy word; the product is masked by
-    // multiplication so that it is computed (and its loads issued) unconditionally
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const bool act = qi[u][j] >= 0;
-            atomicAdd(act ? &q[qi[u][j]] : &qdummy[lane], prod[u][j] * (act ? 1.0 : 0.0));
-        }
+}
+
+// One pipeline step over three rotating register sets: a holds batch j (values and
+// gathers issued earlier), b batch j+1 (headers, indices and values), c batch j+2
+// (headers).  Issues b's gathers, c's indices and values, accumulates a, then reloads
+// a's headers with batch j+3 (first chunk cb3).
+template <int CLS, int U, bool DYN>
+__device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> &a, SpmvStage<CLS, U> &b,
+                                          SpmvStage<CLS, U> &c, int cb3, int last, int row0, int sub, int lane,
+                                          const double *__restrict__ x64, const float *__restrict__ x32, double *q,
+                                          double *qdummy)
+{
+    spmv_load_gathers<CLS, U, DYN>(A, b, sub, x64, x32);
+    spmv_load_entries<CLS, U>(A, c, sub);
+    spmv_accumulate<CLS, U, DYN>(A, a, row0, sub, lane, x32, q, qdummy);
+    spmv_load_headers<CLS, U>(A, a, cb3, last);
 }
 
 // Class-CLS chunks [c0, c1) (headers sorted by class): q[row - row0] += a * x.
@@ -366,12 +404,14 @@ __device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> 
 //   that is lower, reading the value copy of that precision; dynamic class 4 skips it.
 // Four lanes share a chunk, each owning 4 consecutive entries: one 4-byte load of entry
 // indices and one aligned 4-value load per lane, so a wave instruction covers 16
-// chunks.  Three-stage software pipeline over batches of 16U chunks per wave: while
-// batch j's x gathers are in flight, batch j+1's indices and values (its headers
-// arrived a batch earlier) and batch j+2's headers are loaded.  Two register sets
-// alternate (the loop body is unrolled twice), so no register with a load in flight is
-// ever copied; every load is unconditional (clamped header index, padded per-entry
-// arrays), so the loop has no branch for the compiler to sink a load into.
+// chunks.  Four-stage software pipeline over batches of 16U chunks per wave: batch j is
+// accumulated while batch j+1's x gathers, batch j+2's indices and values and batch
+// j+3's headers are in flight, so every wait is for loads issued a whole step earlier.
+// Three register sets rotate (the loop body is unrolled three times), so no register
+// with a load in flight is ever copied; every load is unconditional (clamped header
+// index, padded per-entry arrays), so the loop has no branch for the compiler to sink a
+// load into; the loop exit is a scalar test after each group of three batches (the last
+// batches of a group may lie past the end: clamped loads, every entry inactive).
 template <int CLS, int U, bool DYN>
 __device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, long long c1, int row0,
                                            const double *__restrict__ x64, const float *__restrict__ x32,
@@ -386,18 +426,19 @@ __device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, lon
     const int step = nwv * 16 * U;
     const int last = (int)c1 - 1;
     int bw = (int)c0 + wv * 16 * U;
-    SpmvStage<CLS, U> X, Y;
-    spmv_load_headers<CLS, U>(A, X, bw + cs, last);
-    spmv_load_headers<CLS, U>(A, Y, bw + step + cs, last);
-    spmv_load_entries<CLS, U>(A, X, sub);
     if (bw > last) return;
-    // batches go in pairs with a single exit test after both, so no load for a later
-    // batch can be moved behind an exit branch; the second batch of the last pair may
-    // lie past the end (clamped loads, every entry inactive)
+    SpmvStage<CLS, U> S0, S1, S2;
+    spmv_load_headers<CLS, U>(A, S0, bw + cs, last);
+    spmv_load_headers<CLS, U>(A, S1, bw + step + cs, last);
+    spmv_load_headers<CLS, U>(A, S2, bw + 2 * step + cs, last);
+    spmv_load_entries<CLS, U>(A, S0, sub);
+    spmv_load_entries<CLS, U>(A, S1, sub);
+    spmv_load_gathers<CLS, U, DYN>(A, S0, sub, x64, x32);
     do {
-        spmv_step<CLS, U, DYN>(A, X, Y, bw + 2 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
-        spmv_step<CLS, U, DYN>(A, Y, X, bw + 3 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
-        bw += 2 * step;
+        spmv_step<CLS, U, DYN>(A, S0, S1, S2, bw + 3 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN>(A, S1, S2, S0, bw + 4 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN>(A, S2, S0, S1, bw + 5 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        bw += 3 * step;
     } while (bw <= last);
 }
 

@@ -51,8 +51,8 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
-        if (a.need64) d64[row] = v[k];
-        if (a.need32) d32[row] = (float)v[k];
+        if (a.need64) store_wt(&d64[row], v[k]);
+        if (a.need32) store_wt(&d32[row], (float)v[k]);
     }
 }
 
@@ -65,7 +65,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+        if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs[R.loc(k)]);
     double dummy[1] = {0.0};
     if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
     zero_q<RPT, BLK>(q, R);
