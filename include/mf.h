@@ -194,6 +194,7 @@ typedef struct mf_matrix_stats {
     double alg_bytes_fp64;     /* M (f64) = sum over tiles of 12 + 9 * nnz */
     double alg_bytes_mixed;    /* M (mixed) = sum over tiles of 12 + (s_v + 1) * nnz */
     double device_bytes;       /* resident device footprint */
+    int64_t positions;         /* entry positions of the device streams (nnz + alignment padding) */
 } mf_matrix_stats;
 
 typedef struct mf_solve_options {

@@ -75,6 +75,7 @@ class MatrixStats(ctypes.Structure):
         ("chunks", ctypes.c_int64), ("tilem", ctypes.c_int), ("tilen", ctypes.c_int),
         ("tiles_by_class", ctypes.c_int64 * 4), ("nnz_by_class", ctypes.c_int64 * 4),
         ("alg_bytes_fp64", ctypes.c_double), ("alg_bytes_mixed", ctypes.c_double), ("device_bytes", ctypes.c_double),
+        ("positions", ctypes.c_int64),
     ]
 
 

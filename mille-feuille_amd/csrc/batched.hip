@@ -396,7 +396,7 @@ static BatchKernel pick_batch(int rpt)
 extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
                                          int use_graph, mf_solve_info *per_system, double *time_ms, void *stream)
 {
-    if (!B || !d_b || !d_x) return MF_ERR_INVALID;
+    if (!B || !d_b || !d_x || !mf::aligned16(d_b) || !mf::aligned16(d_x)) return MF_ERR_INVALID;
     mf_matrix *A = B->M;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;

@@ -182,17 +182,29 @@ template <int K>
 __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bool visibility, int wg,
                          double *scratch)
 {
-    __shared__ double s_bc[kMaxRed];
+    __shared__ double s_part[kMaxRed * 16];  // per-wave partials of this member
+    __shared__ double s_bc[kMaxRed];         // the group result
     __shared__ int s_fail;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     if (visibility) wait_own_stores();
     if (threadIdx.x == 0) s_fail = 0;
-    block_sum<K>(v, scratch);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const double w = wave_sum(v[k]);
+        if (lane == 0) s_part[k * 16 + wv] = w;
+    }
+    __syncthreads();
     epoch++;
     const int G = gs.G;
     const long long t0 = __builtin_readcyclecounter();
-    // wave 0 publishes this member's record; with visibility every wave has waited for
-    // its write-through stores before the barrier in block_sum, so they are complete
-    if (threadIdx.x < K) put_record(rec_at(gs, epoch, threadIdx.x, wg), v[threadIdx.x], epoch);
+    // wave 0 combines the wave partials in a fixed order and publishes this member's
+    // record; with visibility every wave has waited for its write-through stores before
+    // the barrier above, so they are complete
+    if (threadIdx.x < K) {
+        double m = 0.0;
+        for (int i = 0; i < nwv; i++) m += s_part[threadIdx.x * 16 + i];
+        put_record(rec_at(gs, epoch, threadIdx.x, wg), m, epoch);
+    }
     // keep the compiler from moving the record stores behind this thread's polling
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (wg == 0) {
@@ -210,29 +222,35 @@ __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bo
             for (int kk = 0; kk < K; kk++)
                 if (kk == k) acc[kk] += d;
         }
-        block_sum<K>(acc, scratch);  // fixed order: per thread, then waves, then block
-        const bool fail = s_fail != 0;
-        if (threadIdx.x < K)
-            put_record(rec_at(gs, epoch, threadIdx.x, G), acc[threadIdx.x], fail ? (epoch | kAbortTag) : epoch);
+        // fixed order: per thread, then per wave (DPP), then over the waves
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const double w = wave_sum(acc[k]);
+            if (lane == 0) scratch[k * 16 + wv] = w;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) {
+            double t = 0.0;
+            for (int i = 0; i < nwv; i++) t += scratch[threadIdx.x * 16 + i];
+            put_record(rec_at(gs, epoch, threadIdx.x, G), t, s_fail ? (epoch | kAbortTag) : epoch);
+            s_bc[threadIdx.x] = t;
+        }
         // after the broadcast (off the other members' critical path): drop stale L2 lines
         if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] = acc[k];
-        __syncthreads();  // s_fail is reset by the next call
-        return !fail;
+    } else {
+        if (threadIdx.x < K) {
+            double d = 0.0;
+            bool ab = false;
+            if (!poll_record(gs, rec_at(gs, epoch, threadIdx.x, G), epoch, t0, d, true, &ab) || ab) s_fail = 1;
+            s_bc[threadIdx.x] = d;
+        }
+        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    if (threadIdx.x < K) {
-        double d = 0.0;
-        bool ab = false;
-        if (!poll_record(gs, rec_at(gs, epoch, threadIdx.x, G), epoch, t0, d, true, &ab) || ab) s_fail = 1;
-        s_bc[threadIdx.x] = d;
-    }
-    if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = s_bc[k];
     const bool fail = s_fail != 0;
-    __syncthreads();  // s_bc and s_fail are rewritten by the next call
+    __syncthreads();  // s_part, s_bc and s_fail are rewritten by the next call
     return !fail;
 }
 

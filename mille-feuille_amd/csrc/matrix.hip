@@ -495,6 +495,7 @@ extern "C" mf_status mf_matrix_get_stats(const mf_matrix *A, mf_matrix_stats *st
     st->alg_bytes_fp64 = A->L.alg_bytes_f64;
     st->alg_bytes_mixed = A->L.alg_bytes_mixed;
     st->device_bytes = A->L.device_bytes;
+    st->positions = A->L.npos;
     return MF_SUCCESS;
 }
 
@@ -509,7 +510,7 @@ int spmv_grid(const mf_matrix *A)
 extern "C" mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int precision, int transpose,
                              void *stream)
 {
-    if (!A || !d_x || !d_y) return MF_ERR_INVALID;
+    if (!A || !d_x || !d_y || !aligned16(d_x)) return MF_ERR_INVALID;
     if (transpose) return MF_ERR_UNSUPPORTED;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;

@@ -44,4 +44,8 @@ uint16_t f32_to_f16_rne(float v);
 
 int64_t default_threads();
 
+// Device vectors passed to the solvers and the SpMV must be 16-byte aligned (the SpMV
+// reads 4 values per lane with one aligned load; hipMalloc and torch allocations are).
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
 }  // namespace mf

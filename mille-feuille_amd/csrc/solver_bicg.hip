@@ -361,7 +361,7 @@ using namespace mf;
 extern "C" mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                                        mf_solve_info *info, void *stream)
 {
-    if (!A || !d_b || !d_x) return MF_ERR_INVALID;
+    if (!A || !d_b || !d_x || !aligned16(d_b) || !aligned16(d_x)) return MF_ERR_INVALID;
     if (A->L.n_rows != A->L.n_cols) return MF_ERR_INVALID;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     mf_solve_options o;

@@ -206,7 +206,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
 extern "C" mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                                  mf_solve_info *info, void *stream)
 {
-    if (!A || !d_b || !d_x) return MF_ERR_INVALID;
+    if (!A || !d_b || !d_x || !aligned16(d_b) || !aligned16(d_x)) return MF_ERR_INVALID;
     if (A->L.n_rows != A->L.n_cols) return MF_ERR_INVALID;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     mf_solve_options o;
@@ -241,7 +241,7 @@ extern "C" mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, 
 extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
                                     int *best_variant, double *best_ms_per_iter, void *stream)
 {
-    if (!A || !d_b || !d_x) return MF_ERR_INVALID;
+    if (!A || !d_b || !d_x || !aligned16(d_b) || !aligned16(d_x)) return MF_ERR_INVALID;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
     const bool mixed = precision == MF_PREC_MIXED && A->L.has_lowp;
