@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
+#include "sell.h"
 
 namespace mfk {
 
@@ -27,6 +28,7 @@ struct CgArgs {
     int dyn_mode;
     double *p64b;    // second publish buffer pair (pipelined variant)
     float *p32b;
+    DevSell S;       // sliced-ELL streams (kernels instantiated with SELL)
 };
 
 enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
@@ -71,6 +73,8 @@ struct CgPart {
     int n_sys;       // rows of the system (for the RMS of p)
     double *p64b;    // second publish buffer pair (pipelined variant)
     float *p32b;
+    DevSell S;       // sliced-ELL streams (SELL kernels) and this wave's slot in them
+    int slot;
 };
 
 // This workgroup's share of x: in LDS (XREG = false) or in registers (XREG = true).
@@ -89,6 +93,34 @@ struct XVec {
         else lds[loc] = v;
     }
 };
+
+// The SpMV result q = A v for this thread's rows: in registers (sliced-ELL SpMV, SELL =
+// true) or in the workgroup's LDS slice (tile-chunk SpMV).
+template <int RPT, bool SELL>
+struct QVec {
+    double reg[SELL ? RPT : 1];
+    double *lds;
+    __device__ __forceinline__ double get(int k, int loc) const
+    {
+        if constexpr (SELL) return reg[k];
+        else return lds[loc];
+    }
+};
+
+// q <- A v (tile classes when MIXED) for the vector v published in x64/x32.
+template <bool MIXED, int U, bool DYN, bool SELL, int RPT, int BLK>
+__device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, const double *x64, const float *x32,
+                                        QVec<RPT, SELL> &qv, double *scratch)
+{
+    if constexpr (SELL) {
+        spmv_sell<MIXED, RPT, BLK>(a.S, a.slot, R.row0, x64, x32, qv.reg);
+    } else {
+        zero_q<RPT, BLK>(qv.lds, R);
+        __syncthreads();
+        spmv_chunks<MIXED, U, DYN>(a.A, a.cr, R.row0, x64, x32, qv.lds, scratch);
+        __syncthreads();
+    }
+}
 
 struct CgResult {
     int it, repl, status;
@@ -124,8 +156,8 @@ __device__ __forceinline__ void classify_segments(const CgPart &a, Rows<RPT, BLK
 
 // r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old)
 // when fix_p; returns the r'r sum.  One SpMV and two syncs.
-template <int RPT, int BLK, class Sync, bool XREG>
-__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *q, double *scratch,
+template <int RPT, int BLK, class Sync, bool XREG, bool SELL = false>
+__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, QVec<RPT, SELL> &q, double *scratch,
                                  const XVec<RPT, XREG> &xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
 {
     R.refresh();
@@ -134,16 +166,13 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
         if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs.get(k, R.loc(k)));
     double dummy[1] = {0.0};
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
-    zero_q<RPT, BLK>(q, R);
-    __syncthreads();
-    spmv_chunks<false, spmv_unroll<RPT, XREG ? 3 : 2>()>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
-    __syncthreads();
+    spmv_op<false, spmv_unroll<RPT, XREG ? 3 : 2>(), false, SELL>(a, R, a.x, nullptr, q, scratch);
     double part[1] = {0.0};
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const double rt = a.b[R.row0 + R.loc(k)] - q[R.loc(k)];
+        const double rt = a.b[R.row0 + R.loc(k)] - q.get(k, R.loc(k));
         if (fix_p) p[k] += rt - r[k];
         r[k] = rt;
         part[0] += rt * rt;
@@ -157,12 +186,14 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, d
 // The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
 // a.x; every member returns the same CgResult.
 template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = spmv_unroll<RPT, 2>(), bool XREG = false,
-          bool DYN = false>
-__device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
+          bool DYN = false, bool SELL = false>
+__device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q_lds,
                             double *xs_lds)
 {
     XVec<RPT, XREG> xs;
     xs.lds = xs_lds;
+    QVec<RPT, SELL> q;
+    q.lds = q_lds;
     CgResult res;
     res.it = 0;
     res.repl = 0;
@@ -171,12 +202,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     res.rt = -1.0;
     double r[RPT], p[RPT];
     // ---- setup: r = b - A x0, p = r ----
-    if (a.use_x0) {
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<false, U>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
-        __syncthreads();
-    }
+    if (a.use_x0) spmv_op<false, U, false, SELL>(a, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
     R.refresh();
 #pragma unroll
@@ -186,7 +212,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         const int row = R.row0 + R.loc(k);
         const double bk = a.b[row];
         xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
-        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
         p[k] = r[k];
         part[0] += r[k] * r[k];
         part[1] += bk * bk;
@@ -230,7 +256,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             const bool dyn_check = DYN && dyn_on && res.it - dyn_it >= 64;
             if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase || dyn_check)) {
                 const double rr_rec = rr;
-                if (!replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
+                if (!replace_residual<RPT, BLK, Sync, XREG, SELL>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
                     res.status = ST_TIMEOUT;
                     break;
                 }
@@ -243,10 +269,13 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                     // check: return every tile to its static class for the rest of the
                     // solve.  rr, rr_rec and dyn_rr are grid sums, so every workgroup
                     // decides alike.
+                    // Also once half the replacement budget is spent: the checks above
+                    // run only while replacements remain.
                     const bool stalled = dyn_check && rr > 0.25 * dyn_rr;
+                    const bool budget = 2 * res.repl >= a.max_repl;
                     dyn_it = res.it;
                     dyn_rr = rr;
-                    if (dyn_on && !conv && (rr > 4.0 * rr_rec || stalled)) {
+                    if (dyn_on && !conv && (rr > 4.0 * rr_rec || stalled || budget)) {
                         dyn_on = false;
                         R.refresh();
 #pragma unroll
@@ -270,16 +299,13 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             break;
         }
         // q = A p
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<MIXED, U, DYN>(a.A, a.cr, R.row0, a.p64, a.p32, q, scratch);
-        __syncthreads();
+        spmv_op<MIXED, U, DYN, SELL>(a, R, a.p64, a.p32, q, scratch);
         R.refresh();
         if (VAR == 1) {
             double d[1] = {0.0};
 #pragma unroll
             for (int k = 0; k < RPT; k++)
-                if (R.ok(k)) d[0] += p[k] * q[R.loc(k)];
+                if (R.ok(k)) d[0] += p[k] * q.get(k, R.loc(k));
             if (!sy.template sum<1>(d, false, scratch)) {
                 res.status = ST_TIMEOUT;
                 break;
@@ -295,7 +321,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             for (int k = 0; k < RPT; k++) {
                 if (!R.ok(k)) continue;
                 xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
-                r[k] -= alpha * q[R.loc(k)];
+                r[k] -= alpha * q.get(k, R.loc(k));
                 e[0] += r[k] * r[k];
             }
             if (!sy.template sum<1>(e, false, scratch)) {
@@ -326,7 +352,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
                 if (!R.ok(k)) continue;
-                const double qk = q[R.loc(k)];
+                const double qk = q.get(k, R.loc(k));
                 d[0] += p[k] * qk;
                 d[1] += r[k] * qk;
                 d[2] += qk * qk;
@@ -351,7 +377,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                 if (DYN) pold[DYN ? k : 0] = p[k];
                 if (!R.ok(k)) continue;
                 xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
-                r[k] -= alpha * q[R.loc(k)];
+                r[k] -= alpha * q.get(k, R.loc(k));
                 p[k] = r[k] + beta * p[k];
                 e[0] += r[k] * r[k];
                 e[1] += p[k] * p[k];
@@ -378,7 +404,8 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
             pd[k] = p[k];
         }
         double rr_t = 0.0;
-        if (replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false)) res.rt = rr_t;
+        if (replace_residual<RPT, BLK, Sync, XREG, SELL>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false))
+            res.rt = rr_t;
         else res.status = ST_TIMEOUT;
     } else {
         R.refresh();
@@ -410,13 +437,11 @@ __device__ __forceinline__ void publish_buf(const CgPart &a, Rows<RPT, BLK> &R, 
 }
 
 // q <- A v for the vector published in buffer `tb` (tile classes when MIXED).
-template <int RPT, bool MIXED, int BLK, int U>
-__device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int tb, double *q, double *scratch)
+template <int RPT, bool MIXED, int BLK, int U, bool SELL>
+__device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int tb, QVec<RPT, SELL> &q,
+                                         double *scratch)
 {
-    zero_q<RPT, BLK>(q, R);
-    __syncthreads();
-    spmv_chunks<MIXED, U>(a.A, a.cr, R.row0, tb ? a.p64b : a.p64, tb ? a.p32b : a.p32, q, scratch);
-    __syncthreads();
+    spmv_op<MIXED, U, false, SELL>(a, R, tb ? a.p64b : a.p64, tb ? a.p32b : a.p32, q, scratch);
 }
 
 // Pipelined CG (Ghysels & Vanroose 2014, unpreconditioned): with w = A r, s = A p and
@@ -435,12 +460,14 @@ __device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int
 // with binary32 products), and before accepting convergence, r = b - A x (binary64
 // stream), w = A r, s = A p and z = A s are recomputed from their definitions, keeping p.  Five row vectors live in registers,
 // so the launcher uses this variant only up to 2 rows per thread.
-template <int RPT, bool MIXED, int BLK, class Sync, int U, bool XREG>
-__device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q,
+template <int RPT, bool MIXED, int BLK, class Sync, int U, bool XREG, bool SELL = false>
+__device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q_lds,
                                       double *xs_lds)
 {
     XVec<RPT, XREG> xs;
     xs.lds = xs_lds;
+    QVec<RPT, SELL> q;
+    q.lds = q_lds;
     CgResult res;
     res.it = 0;
     res.repl = 0;
@@ -452,12 +479,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
     double r[RPT], w[RPT], p[RPT], sv[RPT], z[RPT];
     int tb = 0;
     // ---- setup: r = b - A x0 ----
-    if (a.use_x0) {
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<false, U>(a.A, a.cr, R.row0, a.x, nullptr, q, scratch);
-        __syncthreads();
-    }
+    if (a.use_x0) spmv_op<false, U, false, SELL>(a, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
     R.refresh();
 #pragma unroll
@@ -467,7 +489,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
         const int row = R.row0 + R.loc(k);
         const double bk = a.b[row];
         xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
-        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
         part[0] += r[k] * r[k];
         part[1] += bk * bk;
     }
@@ -494,11 +516,11 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
         rr = 0.0;
         goto finish;
     }
-    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+    spmv_buf<RPT, MIXED, BLK, U, SELL>(a, R, tb, q, scratch);
     tb ^= 1;
     R.refresh();
 #pragma unroll
-    for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+    for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q.get(k, R.loc(k)) : 0.0;
 
     for (;;) {
         // gamma = (r, r), delta = (w, r); the same reduction publishes w
@@ -522,7 +544,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
                 // recompute r, w, s, z from their definitions (p kept), then redo the
                 // reduction with the fresh vectors
                 double rr_new = 0.0;
-                if (!replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr_new, false)) {
+                if (!replace_residual<RPT, BLK, Sync, XREG, SELL>(a, R, sy, q, scratch, xs, r, p, rr_new, false)) {
                     res.status = ST_TIMEOUT;
                     break;
                 }
@@ -532,32 +554,32 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
                     res.status = ST_TIMEOUT;
                     break;
                 }
-                spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                spmv_buf<RPT, MIXED, BLK, U, SELL>(a, R, tb, q, scratch);
                 tb ^= 1;
                 R.refresh();
 #pragma unroll
-                for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                for (int k = 0; k < RPT; k++) w[k] = R.ok(k) ? q.get(k, R.loc(k)) : 0.0;
                 if (!first) {
                     publish_buf<RPT, BLK>(a, R, p, tb, n64, n32);
                     if (!sy.template sum<1>(dummy, true, scratch)) {
                         res.status = ST_TIMEOUT;
                         break;
                     }
-                    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                    spmv_buf<RPT, MIXED, BLK, U, SELL>(a, R, tb, q, scratch);
                     tb ^= 1;
                     R.refresh();
 #pragma unroll
-                    for (int k = 0; k < RPT; k++) sv[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                    for (int k = 0; k < RPT; k++) sv[k] = R.ok(k) ? q.get(k, R.loc(k)) : 0.0;
                     publish_buf<RPT, BLK>(a, R, sv, tb, n64, n32);
                     if (!sy.template sum<1>(dummy, true, scratch)) {
                         res.status = ST_TIMEOUT;
                         break;
                     }
-                    spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+                    spmv_buf<RPT, MIXED, BLK, U, SELL>(a, R, tb, q, scratch);
                     tb ^= 1;
                     R.refresh();
 #pragma unroll
-                    for (int k = 0; k < RPT; k++) z[k] = R.ok(k) ? q[R.loc(k)] : 0.0;
+                    for (int k = 0; k < RPT; k++) z[k] = R.ok(k) ? q.get(k, R.loc(k)) : 0.0;
                 }
                 res.repl++;
                 rbase = rr_new;
@@ -575,7 +597,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
             break;
         }
         // q = A w (w published by the reduction above)
-        spmv_buf<RPT, MIXED, BLK, U>(a, R, tb, q, scratch);
+        spmv_buf<RPT, MIXED, BLK, U, SELL>(a, R, tb, q, scratch);
         tb ^= 1;
         double beta = 0.0, alpha;
         if (first) {
@@ -591,7 +613,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
-            const double qk = R.ok(k) ? q[R.loc(k)] : 0.0;
+            const double qk = R.ok(k) ? q.get(k, R.loc(k)) : 0.0;
             z[k] = qk + beta * z[k];
             sv[k] = w[k] + beta * sv[k];
             p[k] = r[k] + beta * p[k];
@@ -608,7 +630,8 @@ finish:
     // write x, then the binary64 true residual ||b - A x||^2 (r is dead from here on)
     if (res.status != ST_TIMEOUT) {
         double rr_t = 0.0;
-        if (replace_residual<RPT, BLK, Sync, XREG>(a, R, sy, q, scratch, xs, r, p, rr_t, false)) res.rt = rr_t;
+        if (replace_residual<RPT, BLK, Sync, XREG, SELL>(a, R, sy, q, scratch, xs, r, p, rr_t, false))
+            res.rt = rr_t;
         else res.status = ST_TIMEOUT;
     } else {
         R.refresh();
@@ -621,13 +644,15 @@ finish:
     return res;
 }
 
-template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN>
+template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN, bool SELL = false>
 __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
 {
     extern __shared__ double smem[];
     double *scratch = smem;                  // 64 doubles for reductions
-    double *q = smem + 64;                   // rows_cap doubles: SpMV result
-    double *xs = smem + 64 + a.A.rows_cap;   // rows_cap doubles: this workgroup's x (XREG = false)
+    // SELL: the SpMV result stays in registers, x follows the reduction scratch;
+    // otherwise rows_cap doubles of SpMV result, then x (XREG = false)
+    double *q = SELL ? nullptr : smem + 64;
+    double *xs = SELL ? smem + 64 : smem + 64 + a.A.rows_cap;
     const int G = a.gs.G;
     const int wg = logical_wg(blockIdx.x, G);
     const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
@@ -659,12 +684,14 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.n_sys = a.A.n;
     part.p64b = a.p64b;
     part.p32b = a.p32b;
+    part.S = a.S;
+    part.slot = __builtin_amdgcn_readfirstlane(wg * (BLK / 64) + (int)(threadIdx.x >> 6));
     CgResult res;
     if constexpr (VAR == 3)
-        res = cg_core_pipelined<RPT, MIXED, BLK, GridOps, spmv_unroll<RPT, XREG ? 6 : 5>(), XREG>(part, R, sy, scratch,
-                                                                                                 q, xs);
+        res = cg_core_pipelined<RPT, MIXED, BLK, GridOps, spmv_unroll<RPT, XREG ? 6 : 5>(), XREG, SELL>(
+            part, R, sy, scratch, q, xs);
     else
-        res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT, (XREG || DYN) ? 3 : 2>(), XREG, DYN>(
+        res = cg_core<RPT, VAR, MIXED, BLK, GridOps, spmv_unroll<RPT, (XREG || DYN) ? 3 : 2>(), XREG, DYN, SELL>(
             part, R, sy, scratch, q, xs);
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)res.it;

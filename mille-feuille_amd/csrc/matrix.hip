@@ -49,14 +49,27 @@ mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s)
 
 DeviceLayout::~DeviceLayout() { release(); }
 
+static void free_sell(SellLayout *SL)
+{
+    if (!SL) return;
+    for (int k = 0; k < mfk::kSellStreams; k++) {
+        free_dev(SL->soff[k]);
+        free_dev(SL->idx[k]);
+        free_dev(SL->val[k]);
+    }
+    delete SL;
+}
+
 void free_partition(Partition &P)
 {
     free_dev(P.wg_tr);
     free_dev(P.wg_chunk);
     free_dev(P.chunks);
+    free_sell(P.sell);
     P.wg_tr = nullptr;
     P.wg_chunk = nullptr;
     P.chunks = nullptr;
+    P.sell = nullptr;
 }
 
 void DeviceLayout::release()
@@ -129,7 +142,9 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
         v16.assign(P, 0);
         v8.assign(P, 0);
     }
-#pragma omp parallel for schedule(dynamic, 64)
+    L.h_ev.assign(ts.nnz, 0.0);
+    int max_off = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : max_off)
     for (int bi = 0; bi < ts.tilem; bi++) {
         int64_t c = L.tr_chunk[bi], pos = tr_pos[bi];
         for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++) {
@@ -142,6 +157,9 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
                     const int64_t e = e0 + j, q = pos + j;
                     eid[q] = (uint8_t)((ts.lrow[e] << 4) | ts.lcol[e]);
                     const double v = val[ts.src[e]];
+                    L.h_ev[e] = v;
+                    const int off = std::abs((ts.tile_col[t] * kTile + ts.lcol[e]) - (bi * kTile + ts.lrow[e]));
+                    max_off = std::max(max_off, off);
                     v64[q] = v;
                     src32[q] = (int)ts.src[e];
                     if (!v32.empty()) {
@@ -164,6 +182,7 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
             }
         }
     }
+    L.max_offset = max_off;
     L.has_lowp = any_lowp && !v32.empty();
     L.has_copies = !v32.empty();
     L.need_p64_mixed = L.tiles_by_class[MF_FP64] > 0;
@@ -258,6 +277,7 @@ mf_status DeviceLayout::build_ranges(const std::vector<int> &cuts, hipStream_t s
         rows_cap = std::max(rows_cap, std::max(0, r1 - r0));
     }
     P.rows_cap = rows_cap;
+    P.h_tr = cuts;
     mf_status st;
     if ((st = upload((void **)&P.wg_tr, cuts.data(), sizeof(int) * (R + 1), s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&P.wg_chunk, cb.data(), sizeof(long long) * cb.size(), s)) != MF_SUCCESS) return st;
@@ -281,6 +301,185 @@ mfk::DevMatrix DeviceLayout::dev(const Partition &P) const
     d.n = n_rows;
     d.rows_cap = P.rows_cap;
     return d;
+}
+
+bool sell_supported(const mf_matrix *A, int rpt)
+{
+    return A->L.nnz > 0 && (int64_t)A->L.max_offset + 1024 * (int64_t)(rpt - 1) <= 32767 &&
+           A->L.h_ev.size() == (size_t)A->L.nnz;
+}
+
+// Builds the sliced-ELL streams in `mask` (bit s: stream s) for partition P with rpt
+// rows per thread of 1024-thread workgroups: slot (range r, wave v, slice k) covers rows
+// r0 + 1024 k + 64 v + [0, 64).  Pass 1 sizes every slot (longest row of the slice in
+// each stream, in units of E entries), pass 2 writes the entries of each row in tile
+// order -- column tiles ascending, so columns ascending -- at its lane, and the padding:
+// zero values at column offset 0, or at the last row for lanes past the matrix.
+static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &SL, unsigned mask, hipStream_t s)
+{
+    const TileStruct &ts = A->ts;
+    const std::vector<uint8_t> &cls = A->cls;
+    const DeviceLayout &L = A->L;
+    const int R = P.G, NW = 1024 / 64, rpt = SL.rpt, n = L.n_rows;
+    const size_t slots = (size_t)R * NW * rpt;
+    std::vector<int> nu[mfk::kSellStreams];
+    for (int k = 0; k < mfk::kSellStreams; k++)
+        if (mask >> k & 1) nu[k].assign(slots, 0);
+    // visits the entries of the slice starting at row rs of range r: f(stream class, lane, entry, row, col)
+    auto visit = [&](int r, int rs, auto &&f) {
+        const int tr_end = P.h_tr[r + 1];
+        for (int bi = rs / kTile; bi < std::min(rs / kTile + 4, tr_end); bi++)
+            for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+                for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++) {
+                    const int row = bi * kTile + ts.lrow[e];
+                    f((int)cls[t], row - rs, e, row, ts.tile_col[t] * kTile + ts.lcol[e]);
+                }
+    };
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int r = 0; r < R; r++) {
+        const int r0 = P.h_tr[r] * kTile, r1 = std::min(P.h_tr[r + 1] * kTile, n);
+        for (int v = 0; v < NW; v++)
+            for (int k = 0; k < rpt; k++) {
+                const int rs = r0 + 1024 * k + 64 * v;
+                if (rs >= r1) continue;
+                int cnt[mfk::kSellStreams][64];
+                memset(cnt, 0, sizeof(cnt));
+                visit(r, rs, [&](int c, int l, int64_t, int, int) {
+                    cnt[c][l]++;
+                    cnt[mfk::kSellAll64][l]++;
+                });
+                const size_t slot = ((size_t)r * NW + v) * rpt + k;
+                for (int st = 0; st < mfk::kSellStreams; st++) {
+                    if (!(mask >> st & 1)) continue;
+                    int m = 0;
+                    for (int l = 0; l < 64; l++) m = std::max(m, cnt[st][l]);
+                    const int E = mfk::sell_e_host(st);
+                    nu[st][slot] = (m + E - 1) / E;
+                }
+            }
+        // an empty slice between two non-empty ones of a slot gets one zero unit (the
+        // SpMV ends at most one slice per unit)
+        for (int st = 0; st < mfk::kSellStreams; st++) {
+            if (!(mask >> st & 1)) continue;
+            for (int v = 0; v < NW; v++) {
+                int *sl = nu[st].data() + ((size_t)r * NW + v) * rpt;
+                int k0 = -1, k1 = -1;
+                for (int k = 0; k < rpt; k++)
+                    if (sl[k] > 0) {
+                        if (k0 < 0) k0 = k;
+                        k1 = k;
+                    }
+                for (int k = k0 + 1; k0 >= 0 && k < k1; k++) sl[k] = std::max(sl[k], 1);
+            }
+        }
+    }
+    std::vector<int> soff[mfk::kSellStreams];
+    std::vector<int16_t> idx[mfk::kSellStreams];
+    std::vector<uint8_t> val[mfk::kSellStreams];
+    for (int st = 0; st < mfk::kSellStreams; st++) {
+        if (!(mask >> st & 1)) continue;
+        soff[st].assign(slots + 1, 0);
+        int64_t u = 0;
+        for (size_t i = 0; i < slots; i++) {
+            soff[st][i] = (int)u;
+            u += nu[st][i];
+        }
+        soff[st][slots] = (int)u;
+        if (u >= INT32_MAX / 64) return MF_ERR_TOO_LARGE;
+        SL.units[st] = u;
+        const int E = mfk::sell_e_host(st);
+        idx[st].assign((size_t)u * 64 * E, 0);
+        val[st].assign((size_t)u * 64 * E * mfk::sell_vbytes_host(st), 0);
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int r = 0; r < R; r++) {
+        const int r0 = P.h_tr[r] * kTile, r1 = std::min(P.h_tr[r + 1] * kTile, n);
+        for (int v = 0; v < NW; v++)
+            for (int k = 0; k < rpt; k++) {
+                const int rs = r0 + 1024 * k + 64 * v;
+                if (rs >= r1) continue;
+                const size_t slot = ((size_t)r * NW + v) * rpt + k;
+                // column words are offsets from the lane's slice-0 row rs - 1024 k + l;
+                // padding points at the lane's own row, or at the last row past the matrix
+                const int base = rs - 1024 * k;
+                for (int st = 0; st < mfk::kSellStreams; st++) {
+                    if (!(mask >> st & 1)) continue;
+                    const int E = mfk::sell_e_host(st);
+                    for (int l = 0; l < 64; l++) {
+                        const int16_t pad = (int16_t)(std::min(rs + l, n - 1) - (base + l));
+                        for (int uu = soff[st][slot]; uu < soff[st][slot + 1]; uu++)
+                            for (int j = 0; j < E; j++) idx[st][((size_t)uu * 64 + l) * E + j] = pad;
+                    }
+                }
+                int pos[mfk::kSellStreams][64];
+                memset(pos, 0, sizeof(pos));
+                visit(r, rs, [&](int c, int l, int64_t e, int row, int col) {
+                    const double x = L.h_ev[e];
+                    for (int st : {c, (int)mfk::kSellAll64}) {
+                        if (!(mask >> st & 1)) continue;
+                        const int E = mfk::sell_e_host(st);
+                        const int p = pos[st][l]++;
+                        const size_t i = ((size_t)(soff[st][slot] + p / E) * 64 + l) * E + p % E;
+                        idx[st][i] = (int16_t)(col - (row - 1024 * k));
+                        uint8_t *d = val[st].data() + i * mfk::sell_vbytes_host(st);
+                        if (st == 0 || st == mfk::kSellAll64) {
+                            memcpy(d, &x, 8);
+                        } else if (st == 1) {
+                            const float f = (float)x;
+                            memcpy(d, &f, 4);
+                        } else if (st == 2) {
+                            const uint16_t h = f64_to_f16_rne(x);
+                            memcpy(d, &h, 2);
+                        } else {
+                            d[0] = (uint8_t)(int8_t)x;
+                        }
+                    }
+                });
+            }
+    }
+    mf_status st = MF_SUCCESS;
+    for (int k = 0; k < mfk::kSellStreams && st == MF_SUCCESS; k++) {
+        if (!(mask >> k & 1)) continue;
+        if ((st = upload((void **)&SL.soff[k], soff[k].data(), sizeof(int) * soff[k].size(), s)) != MF_SUCCESS) break;
+        if ((st = upload(&SL.idx[k], idx[k].data(), sizeof(int16_t) * idx[k].size(), s)) != MF_SUCCESS) break;
+        st = upload(&SL.val[k], val[k].data(), val[k].size(), s);
+        SL.bytes += (double)sizeof(int) * soff[k].size() + 2.0 * idx[k].size() + (double)val[k].size();
+    }
+    if (st == MF_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = MF_ERR_HIP;
+    return st;
+}
+
+mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out)
+{
+    // the sliced-ELL streams are a cache of the partition, rebuilt with it
+    Partition &P = const_cast<Partition &>(*Pc);
+    if (P.sell && P.sell->rpt != rpt) {
+        free_sell(P.sell);
+        P.sell = nullptr;
+    }
+    if (!P.sell) {
+        P.sell = new SellLayout();
+        P.sell->rpt = rpt;
+    }
+    SellLayout &SL = *P.sell;
+    unsigned mask = 0;
+    if (!SL.soff[mfk::kSellAll64]) mask |= 1u << mfk::kSellAll64;
+    if (mixed && !SL.mixed) mask |= 0xFu;
+    if (mask) {
+        mf_status st = build_sell(A, P, SL, mask, s);
+        if (st != MF_SUCCESS) {
+            free_sell(P.sell);
+            P.sell = nullptr;
+            return st;
+        }
+        if (mask & 0xFu) SL.mixed = true;
+    }
+    for (int k = 0; k < mfk::kSellStreams; k++) {
+        out->s[k].soff = SL.soff[k];
+        out->s[k].idx = SL.idx[k];
+        out->s[k].val = SL.val[k];
+    }
+    return MF_SUCCESS;
 }
 
 mf_status Workspace::ensure(int64_t n, int G, hipStream_t s)

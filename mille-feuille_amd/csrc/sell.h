@@ -1,0 +1,241 @@
+// Sliced-ELL value streams of the persistent CG kernel's SpMV (row per lane).
+//
+// Thread t of workgroup w owns rows row0 + t + BLK k (k < RPT), the same rows whose
+// r, p and x it keeps.  The 64 lanes of wave v therefore hold, for each k, a slice of 64
+// consecutive rows row0 + BLK k + 64 v + lane.  Each slice stores, per stream, its rows'
+// entries as units of E entries per lane: unit u holds for lane l the l-th row's entries
+// u*E .. u*E+E-1 of that stream, padded with zero values to the slice's longest row.
+// Every lane thus sums its own row in a register:
+//   * no LDS atomics and no LDS copy of q: the result lands in the thread that owns the
+//     row, which uses it for the dot products and the vector update;
+//   * for banded matrices the 64 lanes gather 64 consecutive entries of x per entry slot,
+//     i.e. whole cache lines;
+//   * column words are 16-bit offsets from the lane's slice-0 row (col - row + BLK k), so
+//     the index stream costs 2 B per entry and the gathers do not depend on the slice;
+//     matrices where that offset leaves the int16 range use the tile-chunk layout.
+// Streams 0..3 hold the entries of tiles of storage class 0..3 (binary64, binary32,
+// binary16, int8 values) for the tile-mixed SpMV; stream 4 holds every entry in binary64
+// for the all-FP64 SpMV and the residual replacement.  A row's sum runs over stream 0,
+// 1, 2, 3 (or 4) in order, each in column order: deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace mfk {
+
+constexpr int kSellStreams = 5;
+constexpr int kSellAll64 = 4;
+
+struct SellStream {
+    const int *soff;   // first unit of each (range, wave, slice) slot, slots in that order, + end
+    const void *idx;   // [unit][64][E] int16 offsets of the column from the slice-0 row
+    const void *val;   // [unit][64][E] values at the stream's width
+};
+
+struct DevSell {
+    SellStream s[kSellStreams];
+};
+
+// entries per lane per unit, value bytes and arithmetic class of each stream
+template <int S> constexpr int sell_e() { return S == 0 || S == 4 ? 2 : S == 3 ? 8 : 4; }
+template <int S> constexpr int sell_vbytes() { return S == 0 || S == 4 ? 8 : S == 1 ? 4 : S == 2 ? 2 : 1; }
+template <int S> constexpr int sell_cls() { return S == 4 ? 0 : S; }
+inline int sell_e_host(int s) { return s == 0 || s == 4 ? 2 : s == 3 ? 8 : 4; }
+inline int sell_vbytes_host(int s) { return s == 0 || s == 4 ? 8 : s == 1 ? 4 : s == 2 ? 2 : 1; }
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int NW>
+struct Words {
+    unsigned w[NW];
+};
+
+// Global-memory view of a stream pointer: the pointers travel through argument structs,
+// where the compiler loses their address space and would emit flat loads (which also
+// count against lgkmcnt and defeat the load pipeline's partial waits).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const void *p)
+{
+    return (const __attribute__((address_space(1))) T *)(p);
+}
+
+// 32-bit words i*NW .. i*NW+NW-1 of `base` with one aligned load (NW = 1, 2 or 4).
+template <int NW>
+__device__ __forceinline__ Words<NW> load_words(const void *base, size_t i)
+{
+    Words<NW> r;
+    if constexpr (NW == 1) {
+        r.w[0] = gptr<unsigned>(base)[i];
+    } else if constexpr (NW == 2) {
+        const u32x2 t = gptr<u32x2>(base)[i];
+        r.w[0] = t.x;
+        r.w[1] = t.y;
+    } else {
+        static_assert(NW == 4, "unit word count");
+        const u32x4 t = gptr<u32x4>(base)[i];
+        r.w[0] = t.x;
+        r.w[1] = t.y;
+        r.w[2] = t.z;
+        r.w[3] = t.w;
+    }
+    return r;
+}
+
+// Pipeline registers of one unit: column words, value words, gathered x.
+template <int S>
+struct SellUnit {
+    static constexpr int E = sell_e<S>();
+    static constexpr int NWI = E / 2;                       // 2 B per column offset
+    static constexpr int NWV = E * sell_vbytes<S>() / 4;
+    using XT = std::conditional_t<sell_cls<S>() == 0, double, float>;
+    Words<NWI> d;
+    Words<NWV> v;
+    XT xg[E];
+};
+
+template <int S>
+__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S> &U, int u, int lane)
+{
+    const size_t i = (size_t)u * 64 + lane;
+    U.d = load_words<SellUnit<S>::NWI>(st.idx, i);
+    U.v = load_words<SellUnit<S>::NWV>(st.val, i);
+}
+
+template <int S>
+__device__ __forceinline__ void sell_gather(SellUnit<S> &U, int row, const double *__restrict__ x64,
+                                            const float *__restrict__ x32)
+{
+#pragma unroll
+    for (int j = 0; j < SellUnit<S>::E; j++) {
+        const int col = row + (int)(short)(U.d.w[j >> 1] >> (16 * (j & 1)));
+        if constexpr (sell_cls<S>() == 0) U.xg[j] = gptr<double>(x64)[col];
+        else U.xg[j] = gptr<float>(x32)[col];
+    }
+}
+
+// Sum of the unit's products: binary64 for class 0, binary32 products added in binary64
+// otherwise (the tile classes' arithmetic of the chunk SpMV).
+template <int S>
+__device__ __forceinline__ double sell_products(const SellUnit<S> &U)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < SellUnit<S>::E; j++) {
+        if constexpr (sell_cls<S>() == 0) {
+            s += __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]) * U.xg[j];
+        } else if constexpr (S == 1) {
+            s += (double)(__uint_as_float(U.v.w[j]) * U.xg[j]);
+        } else if constexpr (S == 2) {
+            const unsigned short h = (unsigned short)(U.v.w[j >> 1] >> (16 * (j & 1)));
+            s += (double)((float)__builtin_bit_cast(_Float16, h) * U.xg[j]);
+        } else {
+            s += (double)((float)(signed char)(U.v.w[j >> 2] >> (8 * (j & 3))) * U.xg[j]);
+        }
+    }
+    return s;
+}
+
+// q[k] += (A x)[row0 + BLK k + 64 v + lane] over stream S's entries.  Column words are
+// offsets from the lane's slice-0 row rowb = row0 + 64 v + lane, so the gathers do not
+// depend on the slice.
+// Three-stage software pipeline over the wave's units: unit i+2's column and value words
+// are loaded and unit i+1's x gathered while unit i is summed into acc.  Three register
+// sets rotate (the loop body is unrolled three times), so no register with a load in
+// flight is copied, and the loop has a single exit test per three steps and no other
+// branch, so the compiler keeps each load where it is issued (a branch between a load and
+// its use lets it sink the load next to the use and wait for it there).  Steps past the
+// last unit load the last unit again and add nothing.
+// The slice being summed is always q[0]: when unit i ends it, acc is added to q[0] and q
+// rotates down by one (selects, no branch).  The layout gives every empty slice between
+// two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
+// and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
+// q's order.  Unit bounds are wave-uniform (scalar loads).
+template <int S, int RPT, int BLK>
+__device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
+                                            const float *__restrict__ x32, double (&q)[RPT])
+{
+    const int lane = threadIdx.x & 63;
+    const int *sb = st.soff + (size_t)slot * RPT;
+    const int u0 = __builtin_amdgcn_readfirstlane(sb[0]);
+    const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
+    if (u0 > last) return;
+    // ends of the slices not yet closed, in scalar registers (e[0]: the current one)
+    int e[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) e[k] = __builtin_amdgcn_readfirstlane(sb[k + 1]);
+    int rot = 0;
+    auto close_slice = [&]() {
+        const double t = q[0];
+#pragma unroll
+        for (int k = 0; k + 1 < RPT; k++) {
+            q[k] = q[k + 1];
+            e[k] = e[k + 1];
+        }
+        q[RPT - 1] = t;
+        rot++;
+    };
+    while (rot < RPT - 1 && e[0] <= u0) close_slice();  // leading empty slices
+    SellUnit<S> S0, S1, S2;
+    double acc = 0.0;
+    int i = u0;
+    sell_load<S>(st, S0, i, lane);
+    sell_load<S>(st, S1, min(i + 1, last), lane);
+    sell_gather<S>(S0, rowb, x64, x32);
+    // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
+    // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
+    // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
+    auto step = [&](SellUnit<S> &a, SellUnit<S> &nb1, SellUnit<S> &nb2) {
+        sell_load<S>(st, nb2, min(i + 2, last), lane);
+        __builtin_amdgcn_sched_barrier(0);
+        sell_gather<S>(nb1, rowb, x64, x32);
+        __builtin_amdgcn_sched_barrier(0);
+        const double pr = sell_products<S>(a);
+        acc += i <= last ? pr : 0.0;
+        const bool cl = i + 1 == e[0];  // unit i ends the current slice
+        const double q0 = q[0] + acc;
+#pragma unroll
+        for (int k = 0; k + 1 < RPT; k++) {
+            q[k] = cl ? q[k + 1] : q[k];
+            e[k] = cl ? e[k + 1] : e[k];
+        }
+        q[RPT - 1] = cl ? q0 : q[RPT - 1];
+        acc = cl ? 0.0 : acc;
+        rot += cl ? 1 : 0;
+        i++;
+    };
+    // scheduling barriers keep each step's loads in issue order, so every wait is for a
+    // load issued a step earlier
+    do {
+        step(S0, S1, S2);
+        __builtin_amdgcn_sched_barrier(0);
+        step(S1, S2, S0);
+        __builtin_amdgcn_sched_barrier(0);
+        step(S2, S0, S1);
+        __builtin_amdgcn_sched_barrier(0);
+    } while (i <= last);
+    while (rot < RPT) close_slice();  // trailing empty slices
+}
+
+// q[k] = (A x) of this thread's rows.  MIXED: streams 0..3 (tile classes, binary32 x for
+// classes 1..3); otherwise stream 4 (binary64 values and x).
+template <bool MIXED, int RPT, int BLK>
+__device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int row0, const double *__restrict__ x64,
+                                          const float *__restrict__ x32, double (&q)[RPT])
+{
+    const int rowb = row0 + (int)(threadIdx.x & ~63u) + (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < RPT; k++) q[k] = 0.0;
+    if constexpr (MIXED) {
+        sell_stream<0, RPT, BLK>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK>(D.s[3], slot, rowb, x64, x32, q);
+    } else {
+        sell_stream<4, RPT, BLK>(D.s[4], slot, rowb, x64, x32, q);
+    }
+}
+
+}  // namespace mfk

@@ -57,28 +57,36 @@ using CgKernel = void (*)(mfk::CgArgs);
 
 // Rows per thread and block size: 1024 threads, up to 8 rows per thread (r and p in
 // at most 128 VGPRs per wave, 16 waves per CU).
-template <int VAR, bool MIXED, bool XREG, bool DYN>
+template <int VAR, bool MIXED, bool XREG, bool DYN, bool SELL = false>
 static CgKernel pick_cg(int rpt)
 {
     switch (rpt) {
-    case 1: return mfk::cg_kernel<1, VAR, MIXED, 1024, XREG, DYN>;
-    case 2: return mfk::cg_kernel<2, VAR, MIXED, 1024, XREG, DYN>;
-    case 4: return mfk::cg_kernel<4, VAR, MIXED, 1024, XREG, DYN>;
-    case 8: return mfk::cg_kernel<8, VAR, MIXED, 1024, XREG, DYN>;
+    case 1: return mfk::cg_kernel<1, VAR, MIXED, 1024, XREG, DYN, SELL>;
+    case 2: return mfk::cg_kernel<2, VAR, MIXED, 1024, XREG, DYN, SELL>;
+    case 4: return mfk::cg_kernel<4, VAR, MIXED, 1024, XREG, DYN, SELL>;
+    case 8: return mfk::cg_kernel<8, VAR, MIXED, 1024, XREG, DYN, SELL>;
     default: return nullptr;
     }
 }
 
 // Pipelined variant: five register vectors, instantiated up to 2 rows per thread
 // (more would spill).
-template <bool MIXED, bool XREG>
+template <bool MIXED, bool XREG, bool SELL = false>
 static CgKernel pick_cg3(int rpt)
 {
     switch (rpt) {
-    case 1: return mfk::cg_kernel<1, 3, MIXED, 1024, XREG, false>;
-    case 2: return mfk::cg_kernel<2, 3, MIXED, 1024, XREG, false>;
+    case 1: return mfk::cg_kernel<1, 3, MIXED, 1024, XREG, false, SELL>;
+    case 2: return mfk::cg_kernel<2, 3, MIXED, 1024, XREG, false, SELL>;
     default: return nullptr;
     }
+}
+
+// Sliced-ELL kernels (static classes, x in LDS).
+static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed)
+{
+    if (var == 1) return mixed ? pick_cg<1, true, false, false, true>(rpt) : pick_cg<1, false, false, false, true>(rpt);
+    if (var == 3) return mixed ? pick_cg3<true, false, true>(rpt) : pick_cg3<false, false, true>(rpt);
+    return mixed ? pick_cg<2, true, false, false, true>(rpt) : pick_cg<2, false, false, false, true>(rpt);
 }
 
 // Variants: 1 textbook (3 syncs per iteration), 2 fused reductions (2 syncs),
@@ -141,17 +149,28 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     if (st != MF_SUCCESS) return st;
     int rpt = 0, B = 0;
     if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
-    // x in LDS next to q when both fit the per-workgroup LDS, else x in registers
-    size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    // Static classes: sliced-ELL SpMV with q in registers and x in LDS.  Otherwise the
+    // tile-chunk SpMV: x in LDS next to q when both fit the per-workgroup LDS, else x in
+    // registers.
+    const size_t lds_x = (64 + (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+    // Default: sliced-ELL for tile-mixed solves (its row-per-lane SpMV halves the mixed
+    // SpMV time); tile chunks for all-FP64 solves, whose streams are bandwidth-bound and
+    // where the chunks carry less padding.  layout 1 / 2 force chunks / sliced-ELL.
+    const bool want_sell = o->layout == 2 || (o->layout == 0 && mixed);
+    const bool sell = want_sell && !dyn && sell_supported(A, rpt) && lds_x <= (size_t)A->max_lds;
+    size_t lds = sell ? lds_x : (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     bool xreg = false;
     if (lds > (size_t)A->max_lds) {
         xreg = true;
-        lds = (64 + (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+        lds = lds_x;
     }
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    CgKernel k = cg_kernel_for(rpt, var, mixed, xreg, dyn);
+    CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed) : cg_kernel_for(rpt, var, mixed, xreg, dyn);
     if (!k) return var == 3 ? MF_ERR_UNSUPPORTED : MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
+    mfk::DevSell dsell;
+    memset(&dsell, 0, sizeof(dsell));
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell)) != MF_SUCCESS) return st;
 
     mfk::CgArgs a;
     a.A = A->L.dev(*P);
@@ -179,6 +198,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.dyn_mode = dyn ? (o->dynamic_precision == 2 ? 2 : 1) : 0;
     a.A.dvis = dyn ? A->ws.dvis : nullptr;
     if (dyn) a.need64 = 1;  // dynamic classes may keep any tile at binary64
+    a.S = dsell;
     float ms = 0;
     double h[32];
     if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;

@@ -9,6 +9,7 @@
 
 #include "device_common.h"
 #include "mf_internal.h"
+#include "sell.h"
 
 namespace mf {
 
@@ -16,12 +17,26 @@ namespace mf {
 // headers sorted by storage class: class k chunks of range r are
 // chunks[wg_chunk[4r+k] .. wg_chunk[4r+k+1]), so the SpMV runs one branch-free loop
 // per class.
+// Sliced-ELL streams of one partition for RPT rows per thread (sell.h): stream s has
+// units[s] units; soff[s] holds the first unit of every (range, wave, slice) slot.
+struct SellLayout {
+    int rpt = 0;
+    bool mixed = false;             // streams 0..3 built (stream 4 always)
+    int64_t units[mfk::kSellStreams] = {0, 0, 0, 0, 0};
+    int *soff[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *idx[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *val[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double bytes = 0;               // device bytes of the built streams
+};
+
 struct Partition {
     int G = 0;                      // number of ranges
     int rows_cap = 0;
     int *wg_tr = nullptr;           // device, G+1
     long long *wg_chunk = nullptr;  // device, 4G+1
     int3 *chunks = nullptr;         // device, class-sorted headers
+    std::vector<int> h_tr;          // host copy of wg_tr
+    SellLayout *sell = nullptr;     // built on first use by a sliced-ELL solve
 };
 
 void free_partition(Partition &P);
@@ -38,6 +53,8 @@ struct DeviceLayout {
     uint16_t *v16 = nullptr;
     int8_t *v8 = nullptr;
     int *src = nullptr;             // per entry: position in the input CSR arrays
+    std::vector<double> h_ev;       // host: entry values in tile order (sliced-ELL build)
+    int max_offset = 0;             // max |col - row| over the entries
     bool has_lowp = false;
     bool has_copies = false;        // v32/v16/v8 hold every entry (lossy where the class is higher)
     bool need_p64_mixed = false;
@@ -57,6 +74,13 @@ struct DeviceLayout {
     mf_status build_ranges(const std::vector<int> &cuts, hipStream_t s, Partition &P) const;
     mfk::DevMatrix dev(const Partition &P) const;
 };
+
+// Whether the persistent CG kernels with rpt rows per thread can use the sliced-ELL
+// streams: every column within the 16-bit offset range of its lane's slice-0 row.
+bool sell_supported(const mf_matrix *A, int rpt);
+// Sliced-ELL streams of partition P for rpt rows per thread (built and cached on first
+// use; `mixed` also builds the per-class streams).
+mf_status ensure_sell(mf_matrix *A, const Partition *P, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out);
 
 // Per-matrix solver scratch (vectors, barrier records, status words).
 struct Workspace {

@@ -101,12 +101,13 @@ def test_spmv_mixed_matches_oracle_emulation(mf, orc, torch, name):
     A.close()
 
 
+@pytest.mark.parametrize("layout", ["auto", "sell"])
 @pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("name", ["poisson64", "poisson2d_ragged", "poisson2d_one_row_tail", "shifted_random_spd"])
-def test_cg_fp64_iterations_and_residual(mf, orc, torch, name, variant):
+def test_cg_fp64_iterations_and_residual(mf, orc, torch, name, variant, layout):
     rp, ci, va, b, xs = _case(mf, name)
     A = mf.TiledMatrix(rp, ci, va)
-    x, info = A.cg(b, tol=1e-10, precision="fp64", variant=variant)
+    x, info = A.cg(b, tol=1e-10, precision="fp64", variant=variant, layout=layout)
     ref = orc.cg(rp, ci, va, b, tol=1e-10)
     assert info["status"] == 0, info
     it, it_ref = info["iterations"], ref["iterations"]
@@ -130,12 +131,13 @@ def test_cg_grid_sizes_agree(mf, orc, torch, grid):
     A.close()
 
 
+@pytest.mark.parametrize("layout", ["auto", "chunks"])
 @pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("name", ["poisson64", "shifted_random_spd"])
-def test_cg_mixed_reaches_fp64_tolerance(mf, orc, torch, name, variant):
+def test_cg_mixed_reaches_fp64_tolerance(mf, orc, torch, name, variant, layout):
     rp, ci, va, b, xs = _case(mf, name)
     A = mf.TiledMatrix(rp, ci, va, class_policy="rowscaled")
-    x, info = A.cg(b, tol=1e-10, precision="mixed", variant=variant)
+    x, info = A.cg(b, tol=1e-10, precision="mixed", variant=variant, layout=layout)
     assert info["status"] == 0, info
     rel = orc.true_relres(rp, ci, va, x.cpu().numpy(), b)
     assert rel <= 1.2e-10, (rel, info)

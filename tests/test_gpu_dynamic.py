@@ -57,6 +57,8 @@ def test_cg_with_dynamic_classes_reaches_tolerance(mf, orc, torch, mode):
     b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 0)
     A = mf.TiledMatrix(rp, ci, va, class_policy="rowscaled")
     x, info = A.cg(b, tol=1e-10, precision="mixed", dynamic_precision=mode)
+    print(f"dynamic mode {mode}: status {info['status']} iterations {info['iterations']} "
+          f"replacements {info['replacements']} relres_true {info['relres_true']:.3e}", flush=True)
     assert info["status"] == 0, info
     assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
     A.close()
