@@ -215,8 +215,9 @@ typedef struct mf_solve_options {
     int use_x0;            /* x holds an initial guess (else x0 = 0) */
     double watchdog_ms;    /* grid-barrier wait bound (0 = default 10000 ms) */
     int transpose;         /* solve with A^T (same tile stream read transposed) */
-    int layout;            /* single-system CG SpMV layout: 0 = default (row-per-lane sliced-ELL
-                              streams for tile-mixed solves, tile chunks for all-FP64 ones);
+    int layout;            /* single-kernel CG / BiCGSTAB SpMV layout: 0 = default (row-per-lane
+                              sliced-ELL streams for tile-mixed CG and for BiCGSTAB, tile
+                              chunks for all-FP64 CG);
                               1 = tile chunks; 2 = sliced-ELL.  Sliced-ELL needs static classes
                               and every column within 32767 - 1024 (rows per thread - 1) of
                               its row; otherwise the tile chunks are used. */

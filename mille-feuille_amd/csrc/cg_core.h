@@ -94,19 +94,6 @@ struct XVec {
     }
 };
 
-// The SpMV result q = A v for this thread's rows: in registers (sliced-ELL SpMV, SELL =
-// true) or in the workgroup's LDS slice (tile-chunk SpMV).
-template <int RPT, bool SELL>
-struct QVec {
-    double reg[SELL ? RPT : 1];
-    double *lds;
-    __device__ __forceinline__ double get(int k, int loc) const
-    {
-        if constexpr (SELL) return reg[k];
-        else return lds[loc];
-    }
-};
-
 // q <- A v (tile classes when MIXED) for the vector v published in x64/x32.
 template <bool MIXED, int U, bool DYN, bool SELL, int RPT, int BLK>
 __device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, const double *x64, const float *x32,
@@ -362,6 +349,27 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
                 break;
             }
             if (!(d[0] != 0.0) || !isfinite(d[0])) {
+                if constexpr (DYN) {
+                    // Dynamic classes skipped (or lowered) so many tiles that A p
+                    // degenerated -- the reference's absolute |dp| thresholds skip every
+                    // tile once p is small: static classes from here on, and the
+                    // iteration is redone (p and r are unchanged).  d is a grid sum, so
+                    // every workgroup takes this branch alike.
+                    if (dyn_on) {
+                        dyn_on = false;
+                        R.refresh();
+#pragma unroll
+                        for (int k = 0; k < RPT; k++)
+                            if ((R.loc(k) & 15) == 0 && R.loc(k) < R.nrows)
+                                store_wt(&a.dvis[(R.row0 + R.loc(k)) >> 4], (uint8_t)0);
+                        double z[1] = {0.0};
+                        if (!sy.template sum<1>(z, true, scratch)) {
+                            res.status = ST_TIMEOUT;
+                            break;
+                        }
+                        continue;
+                    }
+                }
                 res.status = ST_BREAK;
                 break;
             }

@@ -219,6 +219,36 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
     while (rot < RPT) close_slice();  // trailing empty slices
 }
 
+// The SpMV result q = A v for a thread's rows: in registers (sliced-ELL SpMV, SELL = true)
+// or in the workgroup's LDS slice (tile-chunk SpMV).
+template <int RPT, bool SELL>
+struct QVec {
+    double reg[SELL ? RPT : 1];
+    double *lds;
+    __device__ __forceinline__ double get(int k, int loc) const
+    {
+        if constexpr (SELL) return reg[k];
+        else return lds[loc];
+    }
+};
+
+// A row vector of a thread's rows kept in registers (REG) or in the workgroup's LDS slice.
+template <int RPT, bool REG>
+struct RowVec {
+    double reg[REG ? RPT : 1];
+    double *lds;
+    __device__ __forceinline__ double get(int k, int loc) const
+    {
+        if constexpr (REG) return reg[k];
+        else return lds[loc];
+    }
+    __device__ __forceinline__ void set(int k, int loc, double v)
+    {
+        if constexpr (REG) reg[k] = v;
+        else lds[loc] = v;
+    }
+};
+
 // q[k] = (A x) of this thread's rows.  MIXED: streams 0..3 (tile classes, binary32 x for
 // classes 1..3); otherwise stream 4 (binary64 values and x).
 template <bool MIXED, int RPT, int BLK>

@@ -20,6 +20,7 @@
 
 #include "device_common.h"
 #include "launch_common.h"
+#include "sell.h"
 #include "mf_internal.h"
 #include "solver_internal.h"
 
@@ -38,6 +39,7 @@ struct BicgArgs {
     double *info;  // [0] iterations [1] r'r [2] b'b [3] true r'r [4] status [5] replacements
     double tol2, delta2;
     int maxit, fixed, use_x0, need64, need32, max_repl;
+    DevSell S;  // sliced-ELL streams (kernels instantiated with SELL)
 };
 
 enum { BST_OK = 0, BST_NOTCONV = 4, BST_BREAK = 5, BST_TIMEOUT = 6 };
@@ -56,11 +58,27 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
     }
 }
 
+// q <- A v (tile classes when MIXED) for the vector published in x64/x32: sliced-ELL
+// streams into registers, or tile chunks into the workgroup's LDS slice.
+template <bool MIXED, int U, bool SELL, int RPT, int BLK>
+__device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr, int slot, Rows<RPT, BLK> &R,
+                                          const double *x64, const float *x32, QVec<RPT, SELL> &q, double *scratch)
+{
+    if constexpr (SELL) {
+        spmv_sell<MIXED, RPT, BLK>(a.S, slot, R.row0, x64, x32, q.reg);
+    } else {
+        zero_q<RPT, BLK>(q.lds, R);
+        __syncthreads();
+        spmv_chunks<MIXED, U>(a.A, cr, R.row0, x64, x32, q.lds, scratch);
+        __syncthreads();
+    }
+}
+
 // r <- b - A x (binary64 stream); returns grid {r'r, rhat'r}.
-template <int RPT, int BLK>
-__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int wg,
-                                   double *q, double *xs, double *scratch, unsigned &epoch, double (&r)[RPT],
-                                   double &rr, double &rho)
+template <int RPT, int BLK, bool SELL>
+__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int slot, int wg,
+                                   QVec<RPT, SELL> &q, double *xs, double *scratch, unsigned &epoch,
+                                   double (&r)[RPT], double &rr, double &rho)
 {
     R.refresh();
 #pragma unroll
@@ -68,17 +86,14 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
         if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs[R.loc(k)]);
     double dummy[1] = {0.0};
     if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
-    zero_q<RPT, BLK>(q, R);
-    __syncthreads();
-    spmv_chunks<false, spmv_unroll<RPT, 4>()>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
-    __syncthreads();
+    bicg_spmv<false, spmv_unroll<RPT, 4>(), SELL>(a, cr, slot, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
-        r[k] = a.b[row] - q[R.loc(k)];
+        r[k] = a.b[row] - q.get(k, R.loc(k));
         part[0] += r[k] * r[k];
         part[1] += a.rhat[row] * r[k];
     }
@@ -88,13 +103,18 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
     return true;
 }
 
-template <int RPT, int MIXED, int BLK>
+template <int RPT, int MIXED, int BLK, bool SELL = false>
 __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
 {
     extern __shared__ double smem[];
     double *scratch = smem;
-    double *q = smem + 64;
-    double *xs = smem + 64 + a.A.rows_cap;
+    // SELL: SpMV results in registers, then x and v = A p in LDS after the reduction
+    // scratch (v leaves the registers to the SpMV pipeline); otherwise rows_cap doubles
+    // of SpMV result, then x, with v in registers
+    QVec<RPT, SELL> q;
+    q.lds = SELL ? nullptr : smem + 64;
+    double *xs = SELL ? smem + 64 : smem + 64 + a.A.rows_cap;
+    double *vbuf = smem + 64 + a.A.rows_cap;
     const int G = a.gs.G;
     const int wg = logical_wg(blockIdx.x, G);
     const int tr0 = a.A.wg_tr[wg], tr1 = a.A.wg_tr[wg + 1];
@@ -104,25 +124,24 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     R.refresh();
     const long long *cr = a.A.wg_chunk + 4 * (size_t)wg;
     constexpr int U = spmv_unroll<RPT, 4>();
+    const int slot = __builtin_amdgcn_readfirstlane(wg * (BLK / 64) + (int)(threadIdx.x >> 6));
     unsigned epoch = 0;
 
-    double r[RPT], p[RPT], v[RPT], sv[RPT];
-    if (a.use_x0) {
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<false, U>(a.A, cr, R.row0, a.x, nullptr, q, scratch);
-        __syncthreads();
-    }
+    double r[RPT], p[RPT], sv[RPT];
+    RowVec<RPT, !SELL> v;
+    v.lds = vbuf;
+    if (a.use_x0) bicg_spmv<false, U, SELL>(a, cr, slot, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-        r[k] = p[k] = v[k] = sv[k] = 0.0;
+        r[k] = p[k] = sv[k] = 0.0;
+        if constexpr (!SELL) v.reg[k] = 0.0;
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
         const double bk = a.b[row];
         xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
-        r[k] = a.use_x0 ? bk - q[R.loc(k)] : bk;
+        r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
         p[k] = r[k];
         a.rhat[row] = r[k];
         part[0] += r[k] * r[k];
@@ -152,7 +171,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         if (!a.fixed) {
             bool conv = rr <= a.tol2 * bb;
             if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                if (!bicg_true_residual<RPT, BLK>(a, R, cr, wg, q, xs, scratch, epoch, r, rr, rho)) {
+                if (!bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, r, rr, rho)) {
                     status = BST_TIMEOUT;
                     break;
                 }
@@ -170,17 +189,15 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
             break;
         }
         // v = A p ; alpha = rho / rhat'v
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, cr, R.row0, a.p64, a.p32, q, scratch);
-        __syncthreads();
+        bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.p64, a.p32, q, scratch);
         double d1[1] = {0.0};
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
             if (!R.ok(k)) continue;
-            v[k] = q[R.loc(k)];
-            d1[0] += a.rhat[R.row0 + R.loc(k)] * v[k];
+            const double vk = q.get(k, R.loc(k));
+            v.set(k, R.loc(k), vk);
+            d1[0] += a.rhat[R.row0 + R.loc(k)] * vk;
         }
         if (!grid_sum<1>(a.gs, d1, epoch, false, wg, scratch)) {
             status = BST_TIMEOUT;
@@ -192,7 +209,8 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         }
         const double alpha = rho / d1[0];
 #pragma unroll
-        for (int k = 0; k < RPT; k++) sv[k] = r[k] - alpha * v[k];
+        for (int k = 0; k < RPT; k++) sv[k] = SELL ? (R.ok(k) ? r[k] - alpha * v.get(k, R.loc(k)) : 0.0)
+                                                   : r[k] - alpha * v.get(k, R.loc(k));
         publish<RPT, BLK>(a, R, sv, a.s64, a.s32);
         double z[1] = {0.0};
         if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
@@ -200,16 +218,13 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
             break;
         }
         // t = A s ; omega = t's / t't ; rho' = rhat's - omega rhat't
-        zero_q<RPT, BLK>(q, R);
-        __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, cr, R.row0, a.s64, a.s32, q, scratch);
-        __syncthreads();
+        bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.s64, a.s32, q, scratch);
         double d4[4] = {0.0, 0.0, 0.0, 0.0};
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
             if (!R.ok(k)) continue;
-            const double tk = q[R.loc(k)];
+            const double tk = q.get(k, R.loc(k));
             const double rh = a.rhat[R.row0 + R.loc(k)];
             d4[0] += tk * sv[k];
             d4[1] += tk * tk;
@@ -236,10 +251,10 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
             if (!R.ok(k)) continue;
-            const double tk = q[R.loc(k)];
+            const double tk = q.get(k, R.loc(k));
             xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
             r[k] = sv[k] - omega * tk;
-            p[k] = r[k] + beta * (p[k] - omega * v[k]);
+            p[k] = r[k] + beta * (p[k] - omega * v.get(k, R.loc(k)));
             e[0] += r[k] * r[k];
         }
         publish<RPT, BLK>(a, R, p, a.p64, a.p32);
@@ -256,7 +271,7 @@ done:
     if (status != BST_TIMEOUT) {
         double rtr[RPT];
         double rr_t = 0.0, rho_t = 0.0;
-        if (bicg_true_residual<RPT, BLK>(a, R, cr, wg, q, xs, scratch, epoch, rtr, rr_t, rho_t)) rt = rr_t;
+        if (bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, rtr, rr_t, rho_t)) rt = rr_t;
         else status = BST_TIMEOUT;
     } else {
         R.refresh();
@@ -280,14 +295,14 @@ namespace mf {
 
 using BicgKernel = void (*)(mfk::BicgArgs);
 
-template <int MIXED>
+template <int MIXED, bool SELL>
 static BicgKernel pick_bicg(int rpt)
 {
     switch (rpt) {
-    case 1: return mfk::bicg_kernel<1, MIXED, 1024>;
-    case 2: return mfk::bicg_kernel<2, MIXED, 1024>;
-    case 4: return mfk::bicg_kernel<4, MIXED, 1024>;
-    case 8: return mfk::bicg_kernel<8, MIXED, 1024>;
+    case 1: return mfk::bicg_kernel<1, MIXED, 1024, SELL>;
+    case 2: return mfk::bicg_kernel<2, MIXED, 1024, SELL>;
+    case 4: return mfk::bicg_kernel<4, MIXED, 1024, SELL>;
+    case 8: return mfk::bicg_kernel<8, MIXED, 1024, SELL>;
     default: return nullptr;
     }
 }
@@ -304,11 +319,19 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     if (st != MF_SUCCESS) return st;
     int rpt = 0, B = 0;
     if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
+    // sliced-ELL by default for both precisions (measured faster for binary64 too: its
+    // kernel keeps v in LDS and spills far less than the tile-chunk kernel)
+    const bool want_sell = o->layout != 1;
+    const bool sell = want_sell && sell_supported(A, rpt);
     const size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    BicgKernel k = mixed ? pick_bicg<1>(rpt) : pick_bicg<0>(rpt);
+    BicgKernel k = sell ? (mixed ? pick_bicg<1, true>(rpt) : pick_bicg<0, true>(rpt))
+                        : (mixed ? pick_bicg<1, false>(rpt) : pick_bicg<0, false>(rpt));
     if (!k) return MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
+    mfk::DevSell dsell;
+    memset(&dsell, 0, sizeof(dsell));
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell)) != MF_SUCCESS) return st;
     mfk::BicgArgs a;
     a.A = A->L.dev(*P);
     a.gs.recs = A->ws.recs;
@@ -332,6 +355,7 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
+    a.S = dsell;
     float ms = 0;
     double h[32];
     if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;
