@@ -209,9 +209,13 @@ typedef struct mf_solve_options {
     int max_replacements;  /* 0 = default 64 */
     int grid;              /* 0 = autotuned; else workgroups (must be co-resident) */
     int block;             /* 0 = default (1024) */
-    int variant;           /* 0 = default; 1 = textbook (3 grid syncs/iteration);
+    int variant;           /* 0 = default (4, or 2 when a workgroup's columns span more than
+                              64 other workgroups or under dynamic classes);
+                              1 = textbook (3 grid syncs/iteration);
                               2 = fused reductions (2 grid syncs/iteration);
-                              3 = pipelined (1 grid sync/iteration) */
+                              3 = pipelined (1 grid sync/iteration);
+                              4 = one grid reduction/iteration, p published with point-to-point
+                                  flags to the workgroups that gather it */
     int use_x0;            /* x holds an initial guess (else x0 = 0) */
     double watchdog_ms;    /* grid-barrier wait bound (0 = default 10000 ms) */
     int transpose;         /* solve with A^T (same tile stream read transposed) */
@@ -277,7 +281,7 @@ mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
 mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                             mf_solve_info *info, void *stream);
 /* Autotuner of the persistent CG launch: times short fixed-iteration solves over
- * candidate workgroup counts and the three variants, caches the fastest for this matrix
+ * candidate workgroup counts and the four variants, caches the fastest for this matrix
  * and precision (used by later solves with grid = 0, variant = 0). */
 mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
                          int *best_variant, double *best_ms_per_iter, void *stream);

@@ -29,6 +29,7 @@ struct CgArgs {
     double *p64b;    // second publish buffer pair (pipelined variant)
     float *p32b;
     DevSell S;       // sliced-ELL streams (kernels instantiated with SELL)
+    NbSync nb;       // publish flags and dependency lists (variant 4)
 };
 
 enum { ST_OK = 0, ST_NOTCONV = 4, ST_BREAK = 5, ST_TIMEOUT = 6 };
@@ -75,6 +76,7 @@ struct CgPart {
     float *p32b;
     DevSell S;       // sliced-ELL streams (SELL kernels) and this wave's slot in them
     int slot;
+    NbSync nb;       // publish flags and dependency lists (variant 4)
 };
 
 // This workgroup's share of x: in LDS (XREG = false) or in registers (XREG = true).
@@ -426,6 +428,162 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     return res;
 }
 
+// CG with one grid reduction per iteration (variant 4): the directly summed r'r rides
+// on the reduction of {p'q, r'q, q'q} after the SpMV, and the publish of p needs no grid
+// barrier, only the flags of the workgroups whose rows this one gathers (nb_wait):
+//   [wait for the owners of my columns]  q = A p
+//   {p'q, r'q, q'q, r'r}                 one grid reduction
+//   convergence / replacement tests on the exact r'r, alpha = r'r / p'q,
+//   beta from r'r_new = r'r - 2 alpha r'q + alpha^2 q'q (as variant 2)
+//   x += alpha p, r -= alpha q, p = r + beta p, publish p, raise my flag
+// The reduction after each SpMV is the barrier that keeps a publish from overwriting p
+// while another workgroup still gathers it.  The tests of an iteration run after its
+// SpMV (whose product is then discarded on exit or replacement), so iteration counts are
+// those of the textbook recurrence.  A replacement (tile-mixed) publishes the corrected
+// p with a grid barrier and redoes the iteration.  Only the single-system kernel (all
+// workgroups of the launch) uses it.
+template <int RPT, bool MIXED, int BLK, int U, bool XREG, bool SELL>
+__device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, double *scratch, double *q_lds,
+                               double *xs_lds)
+{
+    XVec<RPT, XREG> xs;
+    xs.lds = xs_lds;
+    QVec<RPT, SELL> q;
+    q.lds = q_lds;
+    CgResult res;
+    res.it = 0;
+    res.repl = 0;
+    res.status = ST_NOTCONV;
+    res.rr = res.bb = 0.0;
+    res.rt = -1.0;
+    double r[RPT], p[RPT];
+    if (a.use_x0) spmv_op<false, U, false, SELL>(a, R, a.x, nullptr, q, scratch);
+    double part[2] = {0.0, 0.0};
+    R.refresh();
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        r[k] = p[k] = 0.0;
+        if (!R.ok(k)) continue;
+        const int row = R.row0 + R.loc(k);
+        const double bk = a.b[row];
+        xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
+        r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
+        p[k] = r[k];
+        part[0] += r[k] * r[k];
+        part[1] += bk * bk;
+    }
+    publish_p<RPT, BLK>(a, R, p);
+    double rr = 0.0, bb = 0.0, rbase = 0.0;
+    bool alive = sy.template sum<2>(part, true, scratch);
+    if (alive) {
+        rr = part[0];
+        bb = part[1];
+        rbase = rr;
+        if (bb == 0.0) {
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                if (R.ok(k)) xs.set(k, R.loc(k), 0.0);
+            rr = 0.0;
+            res.status = ST_OK;
+        }
+    } else {
+        res.status = ST_TIMEOUT;
+    }
+    bool p_visible = true;  // p published behind a grid barrier (no flag wait needed)
+    unsigned pub = 0;       // flag epoch of the last p published without one
+    while (alive && res.status == ST_NOTCONV) {
+        if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
+            res.status = ST_TIMEOUT;
+            break;
+        }
+        p_visible = false;
+        // q = A p
+        spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!R.ok(k)) continue;
+            const double qk = q.get(k, R.loc(k));
+            d[0] += p[k] * qk;
+            d[1] += r[k] * qk;
+            d[2] += qk * qk;
+            d[3] += r[k] * r[k];
+        }
+        if (!sy.template sum<4>(d, false, scratch)) {
+            res.status = ST_TIMEOUT;
+            break;
+        }
+        rr = d[3];
+        if (!a.fixed) {
+            bool conv = rr <= a.tol2 * bb;
+            if (MIXED && res.repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
+                if (!replace_residual<RPT, BLK, GridOps, XREG, SELL>(a, R, sy, q, scratch, xs, r, p, rr, true)) {
+                    res.status = ST_TIMEOUT;
+                    break;
+                }
+                res.repl++;
+                rbase = rr;
+                conv = rr <= a.tol2 * bb;
+                if (!conv) {
+                    p_visible = true;  // the corrected p went out with a grid barrier
+                    continue;
+                }
+            }
+            if (conv) {
+                res.status = ST_OK;
+                break;
+            }
+        }
+        if (res.it >= a.maxit) {
+            res.status = a.fixed ? ST_OK : ST_NOTCONV;
+            break;
+        }
+        if (!(d[0] != 0.0) || !isfinite(d[0])) {
+            res.status = ST_BREAK;
+            break;
+        }
+        const double alpha = rr / d[0];
+        double rr_new = rr - 2.0 * alpha * d[1] + alpha * alpha * d[2];
+        if (!(rr_new > 0.0)) rr_new = 0.0;
+        const double beta = rr_new / rr;
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!R.ok(k)) continue;
+            xs.set(k, R.loc(k), xs.get(k, R.loc(k)) + alpha * p[k]);
+            r[k] -= alpha * q.get(k, R.loc(k));
+            p[k] = r[k] + beta * p[k];
+        }
+        publish_p<RPT, BLK>(a, R, p);
+        nb_signal(a.nb, sy.wg, ++pub);
+        res.it++;
+    }
+    // write x, then the binary64 true residual ||b - A x||^2 (its first grid barrier also
+    // covers a p published without one)
+    if (res.status != ST_TIMEOUT) {
+        double rtrue[RPT], pd[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            rtrue[k] = r[k];
+            pd[k] = p[k];
+        }
+        double rr_t = 0.0;
+        if (replace_residual<RPT, BLK, GridOps, XREG, SELL>(a, R, sy, q, scratch, xs, rtrue, pd, rr_t, false))
+            res.rt = rr_t;
+        else res.status = ST_TIMEOUT;
+    } else {
+        R.refresh();
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+    }
+    res.rr = rr;
+    res.bb = bb;
+    return res;
+}
+
 // Publishes v (rows of this workgroup) into publish buffer `tb` (0: p64/p32, 1: the
 // second pair), binary64 and/or binary32 as the SpMV that reads it needs.
 template <int RPT, int BLK>
@@ -694,8 +852,11 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.p32b = a.p32b;
     part.S = a.S;
     part.slot = __builtin_amdgcn_readfirstlane(wg * (BLK / 64) + (int)(threadIdx.x >> 6));
+    part.nb = a.nb;
     CgResult res;
-    if constexpr (VAR == 3)
+    if constexpr (VAR == 4)
+        res = cg_core_nb<RPT, MIXED, BLK, spmv_unroll<RPT, XREG ? 3 : 2>(), XREG, SELL>(part, R, sy, scratch, q, xs);
+    else if constexpr (VAR == 3)
         res = cg_core_pipelined<RPT, MIXED, BLK, GridOps, spmv_unroll<RPT, XREG ? 6 : 5>(), XREG, SELL>(
             part, R, sy, scratch, q, xs);
     else

@@ -254,6 +254,63 @@ __device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bo
     return !fail;
 }
 
+// Point-to-point visibility of a published vector.  After its write-through stores of
+// the vector are complete, a workgroup raises its flag to `epoch`; a workgroup whose
+// gathers read rows of workgroups dep_idx[dep_ptr[wg] .. dep_ptr[wg+1]) waits for their
+// flags to reach `epoch` and then acquires (agent scope).  One memory round trip, no
+// root, and a workgroup waits only for the few that own its columns.  The caller must
+// keep a publish from overwriting data a reader still gathers (the CG kernels put a grid
+// reduction between every SpMV and the next publish).  Flags are kFlagStride words
+// apart and zeroed before each launch; epochs start at 1.
+constexpr int kFlagStride = 32;
+
+struct NbSync {
+    unsigned *flags;
+    const int *dep_ptr;
+    const int *dep_idx;
+};
+
+__device__ __forceinline__ void nb_signal(const NbSync &nb, int wg, unsigned epoch)
+{
+    wait_own_stores();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&nb.flags[(size_t)wg * kFlagStride], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave 0 polls the flags (one dependency per lane at a time); false when the watchdog
+// fired or the abort flag is set (which it then sets for everyone).  The workgroup
+// barrier at the end publishes the result; s_nbfail is next written by the following
+// call, after the barriers of the SpMV and grid reduction in between.
+__device__ __forceinline__ bool nb_wait(const GridSync &gs, const NbSync &nb, int wg, unsigned epoch)
+{
+    __shared__ int s_nbfail;
+    if (threadIdx.x < 64) {
+        bool ok = true;
+        const int d0 = nb.dep_ptr[wg], d1 = nb.dep_ptr[wg + 1];
+        const long long t0 = __builtin_readcyclecounter();
+        for (int i = d0 + (int)threadIdx.x; i < d1 && ok; i += 64) {
+            const unsigned *f = &nb.flags[(size_t)nb.dep_idx[i] * kFlagStride];
+            for (;;) {
+                const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((int)(v - epoch) >= 0) break;
+                if (__hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                    __builtin_readcyclecounter() - t0 > gs.timeout) {
+                    __hip_atomic_store(gs.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        const unsigned long long bad = __ballot(!ok);
+        if (threadIdx.x == 0) s_nbfail = bad != 0ull;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    return s_nbfail == 0;
+}
+
 __device__ __forceinline__ float val_lowp(const DevMatrix &A, int cls, long long e)
 {
     if (cls == 1) return A.v32[e];

@@ -11,7 +11,8 @@
 #include "solver_internal.h"
 
 namespace mfk {
-__global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigned *ctl, double *info);
+__global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigned *ctl, double *info,
+                                  unsigned *flags, long long nflags);
 }
 
 namespace mf {
@@ -60,7 +61,8 @@ mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, B, lds) != hipSuccess) return MF_ERR_HIP;
     if (G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
     const long long nrec = mfk::rec_words(G);
-    hipLaunchKernelGGL(mfk::reset_sync_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl, A->ws.info);
+    hipLaunchKernelGGL(mfk::reset_sync_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl, A->ws.info,
+                       A->ws.flags, (long long)mfk::kFlagStride * G);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return MF_ERR_HIP;
     if (hipEventCreate(&e1) != hipSuccess) {

@@ -66,10 +66,13 @@ void free_partition(Partition &P)
     free_dev(P.wg_chunk);
     free_dev(P.chunks);
     free_sell(P.sell);
+    free_dev(P.dep_ptr);
+    free_dev(P.dep_idx);
     P.wg_tr = nullptr;
     P.wg_chunk = nullptr;
     P.chunks = nullptr;
     P.sell = nullptr;
+    P.dep_ptr = P.dep_idx = nullptr;
 }
 
 void DeviceLayout::release()
@@ -278,7 +281,33 @@ mf_status DeviceLayout::build_ranges(const std::vector<int> &cuts, hipStream_t s
     }
     P.rows_cap = rows_cap;
     P.h_tr = cuts;
+    // dependency lists: the ranges owning the column tiles of each range's chunks
+    // (square matrices: column tile c is row tile c)
+    std::vector<std::vector<int>> deps(R);
+    if (n_rows == n_cols) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int r = 0; r < R; r++) {
+            std::vector<int> d;
+            for (int64_t c = tr_chunk[cuts[r]]; c < tr_chunk[cuts[r + 1]]; c++) {
+                const int ct = (int)((unsigned)h_chunks[c].y & mfk::kTileIdxMask);
+                const int o = (int)(std::upper_bound(cuts.begin(), cuts.end(), ct) - cuts.begin()) - 1;
+                if (o != r && o >= 0 && o < R) d.push_back(o);
+            }
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+            deps[r] = std::move(d);
+        }
+    }
+    std::vector<int> dptr(R + 1, 0), didx;
+    for (int r = 0; r < R; r++) {
+        dptr[r + 1] = dptr[r] + (int)deps[r].size();
+        P.max_deps = std::max(P.max_deps, (int)deps[r].size());
+        didx.insert(didx.end(), deps[r].begin(), deps[r].end());
+    }
+    if (didx.empty()) didx.push_back(0);
     mf_status st;
+    if ((st = upload((void **)&P.dep_ptr, dptr.data(), sizeof(int) * dptr.size(), s)) != MF_SUCCESS) return st;
+    if ((st = upload((void **)&P.dep_idx, didx.data(), sizeof(int) * didx.size(), s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&P.wg_tr, cuts.data(), sizeof(int) * (R + 1), s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&P.wg_chunk, cb.data(), sizeof(long long) * cb.size(), s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&P.chunks, sorted.data(), sizeof(int3) * sorted.size(), s)) != MF_SUCCESS) return st;
@@ -505,7 +534,11 @@ mf_status Workspace::ensure(int64_t n, int G, hipStream_t s)
     }
     if (G > cap_g) {
         free_dev(recs);
-        if (hipMalloc((void **)&recs, sizeof(unsigned long long) * mfk::kRecWordsPerMember * (size_t)G) != hipSuccess)
+        free_dev(flags);
+        recs = nullptr;
+        flags = nullptr;
+        if (hipMalloc((void **)&recs, sizeof(unsigned long long) * mfk::kRecWordsPerMember * (size_t)G) != hipSuccess ||
+            hipMalloc((void **)&flags, sizeof(unsigned) * mfk::kFlagStride * (size_t)G) != hipSuccess)
             return MF_ERR_ALLOC;
         cap_g = G;
     }
@@ -532,6 +565,7 @@ Workspace::~Workspace()
     free_dev(s32);
     free_dev(rhat);
     free_dev(recs);
+    free_dev(flags);
     free_dev(ctl);
     free_dev(info);
     free_dev(dvis);

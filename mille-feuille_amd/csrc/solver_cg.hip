@@ -20,6 +20,9 @@
 // variant 3: one grid sync (pipelined CG, see cg_core_pipelined): w = A r, s = A p and
 //            z = A s by recurrence, (r,r) and (w,r) reduced by the sync that publishes
 //            w for the iteration's SpMV.
+// variant 4: one grid reduction ({p'q, r'q, q'q, r'r}) per iteration; the publish of p
+//            waits only for the flags of the workgroups owning this one's columns (see
+//            cg_core_nb).  The default when no workgroup depends on more than 64 others.
 // Mixed precision: the SpMV reads every tile at its storage class with binary32
 // products (p converted to binary32 by its owner when published); x, r, the dot
 // products and scalars stay binary64.  The residual is replaced by b - A x computed
@@ -41,10 +44,14 @@
 
 namespace mfk {
 
-__global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigned *ctl, double *info)
+__global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigned *ctl, double *info,
+                                  unsigned *flags, long long nflags)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         recs[i] = 0ull;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nflags;
+         i += (long long)gridDim.x * blockDim.x)
+        flags[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x < 64) ctl[threadIdx.x] = 0u;
     if (blockIdx.x == 0 && threadIdx.x < 32) info[threadIdx.x] = 0.0;
 }
@@ -54,6 +61,9 @@ __global__ void reset_sync_kernel(unsigned long long *recs, long long n, unsigne
 namespace mf {
 
 using CgKernel = void (*)(mfk::CgArgs);
+
+// Variant 4 by default while no workgroup waits on more than this many others.
+constexpr int kAutoMaxDeps = 64;
 
 // Rows per thread and block size: 1024 threads, up to 8 rows per thread (r and p in
 // at most 128 VGPRs per wave, 16 waves per CU).
@@ -86,11 +96,13 @@ static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed)
 {
     if (var == 1) return mixed ? pick_cg<1, true, false, false, true>(rpt) : pick_cg<1, false, false, false, true>(rpt);
     if (var == 3) return mixed ? pick_cg3<true, false, true>(rpt) : pick_cg3<false, false, true>(rpt);
+    if (var == 4) return mixed ? pick_cg<4, true, false, false, true>(rpt) : pick_cg<4, false, false, false, true>(rpt);
     return mixed ? pick_cg<2, true, false, false, true>(rpt) : pick_cg<2, false, false, false, true>(rpt);
 }
 
 // Variants: 1 textbook (3 syncs per iteration), 2 fused reductions (2 syncs),
-// 3 pipelined (1 sync).  Dynamic classes are instantiated for variant 2 only.
+// 3 pipelined (1 sync), 4 one reduction + point-to-point publish flags.  Dynamic classes
+// are instantiated for variant 2 only.
 static CgKernel cg_kernel_for(int rpt, int var, bool mixed, bool xreg, bool dyn)
 {
     if (dyn) {
@@ -100,10 +112,12 @@ static CgKernel cg_kernel_for(int rpt, int var, bool mixed, bool xreg, bool dyn)
     if (xreg) {
         if (var == 1) return mixed ? pick_cg<1, true, true, false>(rpt) : pick_cg<1, false, true, false>(rpt);
         if (var == 3) return mixed ? pick_cg3<true, true>(rpt) : pick_cg3<false, true>(rpt);
+        if (var == 4) return mixed ? pick_cg<4, true, true, false>(rpt) : pick_cg<4, false, true, false>(rpt);
         return mixed ? pick_cg<2, true, true, false>(rpt) : pick_cg<2, false, true, false>(rpt);
     }
     if (var == 1) return mixed ? pick_cg<1, true, false, false>(rpt) : pick_cg<1, false, false, false>(rpt);
     if (var == 3) return mixed ? pick_cg3<true, false>(rpt) : pick_cg3<false, false>(rpt);
+    if (var == 4) return mixed ? pick_cg<4, true, false, false>(rpt) : pick_cg<4, false, false, false>(rpt);
     return mixed ? pick_cg<2, true, false, false>(rpt) : pick_cg<2, false, false, false>(rpt);
 }
 
@@ -140,13 +154,16 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     const bool dyn = o->precision == MF_PREC_MIXED && o->dynamic_precision != 0 && A->L.has_copies;
     const bool mixed = o->precision == MF_PREC_MIXED && (A->L.has_lowp || dyn);
     const int tv = A->tuned_variant[mixed ? 1 : 0];
-    const int var = dyn ? 2 : (o->variant >= 1 && o->variant <= 3 ? o->variant : (tv >= 1 && tv <= 3 ? tv : 2));
+    // requested variant (0: automatic -- 4 when every workgroup depends on few others,
+    // else 2); dynamic classes run variant 2
+    const int req = dyn ? 2 : (o->variant >= 1 && o->variant <= 4 ? o->variant : (tv >= 1 && tv <= 4 ? tv : 0));
     if (A->L.tilem == 0) return MF_ERR_INVALID;
-    const int G = choose_grid(A, o, ((long long)var << 8) | (mixed ? 1 : 0));
+    const int G = choose_grid(A, o, ((long long)(req ? req : 2) << 8) | (mixed ? 1 : 0));
     if (G > A->L.tilem) return MF_ERR_INVALID;
     const Partition *P;
     mf_status st = A->L.partition(G, s, &P);
     if (st != MF_SUCCESS) return st;
+    const int var = req ? req : (P->max_deps <= kAutoMaxDeps ? 4 : 2);
     int rpt = 0, B = 0;
     if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
     // Static classes: sliced-ELL SpMV with q in registers and x in LDS.  Otherwise the
@@ -199,6 +216,9 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.A.dvis = dyn ? A->ws.dvis : nullptr;
     if (dyn) a.need64 = 1;  // dynamic classes may keep any tile at binary64
     a.S = dsell;
+    a.nb.flags = A->ws.flags;
+    a.nb.dep_ptr = P->dep_ptr;
+    a.nb.dep_idx = P->dep_idx;
     float ms = 0;
     double h[32];
     if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;
@@ -268,7 +288,7 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
     const int cands[] = {8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 512};
     double best = 1e30;
     int bg = 0, bv = 2;
-    for (int var = 1; var <= 3; var++) {
+    for (int var = 1; var <= 4; var++) {
         for (int G : cands) {
             if (G > A->L.tilem || G > 2 * A->num_cus) continue;
             mf_solve_options o;
@@ -294,7 +314,7 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
         }
     }
     if (bg == 0) return MF_ERR_TOO_LARGE;
-    for (int var = 1; var <= 3; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
+    for (int var = 1; var <= 4; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
     A->tuned_variant[mixed ? 1 : 0] = bv;
     if (best_grid) *best_grid = bg;
     if (best_variant) *best_variant = bv;

@@ -37,6 +37,10 @@ struct Partition {
     int3 *chunks = nullptr;         // device, class-sorted headers
     std::vector<int> h_tr;          // host copy of wg_tr
     SellLayout *sell = nullptr;     // built on first use by a sliced-ELL solve
+    // ranges owning the columns of each range's entries (itself excluded):
+    // dep_idx[dep_ptr[r] .. dep_ptr[r+1]), device
+    int *dep_ptr = nullptr, *dep_idx = nullptr;
+    int max_deps = 0;
 };
 
 void free_partition(Partition &P);
@@ -89,6 +93,7 @@ struct Workspace {
     double *p64 = nullptr, *v64 = nullptr, *s64 = nullptr, *rhat = nullptr;
     float *p32 = nullptr, *v32 = nullptr, *s32 = nullptr;
     unsigned long long *recs = nullptr;
+    unsigned *flags = nullptr;  // per-workgroup publish flags (kFlagStride words apart), cap_g of them
     unsigned *ctl = nullptr;
     uint8_t *dvis = nullptr;   // dynamic class per column tile
     double *info = nullptr;

@@ -102,7 +102,7 @@ def test_spmv_mixed_matches_oracle_emulation(mf, orc, torch, name):
 
 
 @pytest.mark.parametrize("layout", ["auto", "sell"])
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["poisson64", "poisson2d_ragged", "poisson2d_one_row_tail", "shifted_random_spd"])
 def test_cg_fp64_iterations_and_residual(mf, orc, torch, name, variant, layout):
     rp, ci, va, b, xs = _case(mf, name)
@@ -119,11 +119,12 @@ def test_cg_fp64_iterations_and_residual(mf, orc, torch, name, variant, layout):
     A.close()
 
 
+@pytest.mark.parametrize("variant", [2, 4])
 @pytest.mark.parametrize("grid", [1, 2, 3, 7, 16])
-def test_cg_grid_sizes_agree(mf, orc, torch, grid):
+def test_cg_grid_sizes_agree(mf, orc, torch, grid, variant):
     rp, ci, va, b, xs = _case(mf, "poisson64")
     A = mf.TiledMatrix(rp, ci, va)
-    x, info = A.cg(b, tol=1e-10, grid=grid)
+    x, info = A.cg(b, tol=1e-10, grid=grid, variant=variant)
     ref = orc.cg(rp, ci, va, b, tol=1e-10)
     assert info["status"] == 0 and info["grid"] == grid
     assert abs(info["iterations"] - ref["iterations"]) <= max(1, int(0.02 * ref["iterations"]))
@@ -132,7 +133,7 @@ def test_cg_grid_sizes_agree(mf, orc, torch, grid):
 
 
 @pytest.mark.parametrize("layout", ["auto", "chunks"])
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["poisson64", "shifted_random_spd"])
 def test_cg_mixed_reaches_fp64_tolerance(mf, orc, torch, name, variant, layout):
     rp, ci, va, b, xs = _case(mf, name)

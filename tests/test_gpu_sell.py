@@ -33,12 +33,13 @@ def test_sell_every_shape_matches_oracle(mf, orc, torch, nx, grid, rpt, precisio
     A.close()
 
 
+@pytest.mark.parametrize("variant", [2, 4])
 @pytest.mark.parametrize("layout", ["sell", "chunks"])
-def test_fixed_iterations_follow_the_oracle(mf, orc, torch, layout):
+def test_fixed_iterations_follow_the_oracle(mf, orc, torch, layout, variant):
     """Seven FP64 iterations on a ragged matrix (n = 1037) reproduce the textbook iterate."""
     rp, ci, va, b, xs = _poisson(mf, 17, 61, seed=3)
     A = mf.TiledMatrix(rp, ci, va)
-    x, info = A.cg(b, max_iter=7, fixed_iterations=True, layout=layout)
+    x, info = A.cg(b, max_iter=7, fixed_iterations=True, layout=layout, variant=variant, grid=5)
     ref = orc.cg(rp, ci, va, b, tol=0.0, maxit=7)
     assert info["status"] == 0 and info["iterations"] == 7
     assert np.allclose(x.cpu().numpy(), ref["x"], rtol=1e-10, atol=1e-13)
