@@ -239,6 +239,9 @@ typedef struct mf_solve_info {
     double time_ms;         /* device time of the solve launch (hipEvent) */
     double bytes_per_iter;  /* algorithmic bytes per iteration (M + 16 n; BiCGSTAB 2M + 32 n) */
     double gbs;             /* bytes_per_iter * iterations / time */
+    int layout;             /* SpMV layout used: 1 tile chunks, 2 sliced-ELL, 3 sliced-ELL with the
+                               narrow unit profile; variant of the CG recurrence in `variant` */
+    int variant;
 } mf_solve_info;
 
 void mf_solve_options_default(mf_solve_options *o);

@@ -97,12 +97,12 @@ struct XVec {
 };
 
 // q <- A v (tile classes when MIXED) for the vector v published in x64/x32.
-template <bool MIXED, int U, bool DYN, bool SELL, int RPT, int BLK>
+template <bool MIXED, int U, bool DYN, int SELL, int RPT, int BLK>
 __device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, const double *x64, const float *x32,
-                                        QVec<RPT, SELL> &qv, double *scratch)
+                                        QVec<RPT, (SELL != 0)> &qv, double *scratch)
 {
-    if constexpr (SELL) {
-        spmv_sell<MIXED, RPT, BLK>(a.S, a.slot, R.row0, x64, x32, qv.reg);
+    if constexpr (SELL != 0) {
+        spmv_sell<MIXED, RPT, BLK, SELL == 2>(a.S, a.slot, R.row0, x64, x32, qv.reg);
     } else {
         zero_q<RPT, BLK>(qv.lds, R);
         __syncthreads();
@@ -145,8 +145,8 @@ __device__ __forceinline__ void classify_segments(const CgPart &a, Rows<RPT, BLK
 
 // r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old)
 // when fix_p; returns the r'r sum.  One SpMV and two syncs.
-template <int RPT, int BLK, class Sync, bool XREG, bool SELL = false>
-__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, QVec<RPT, SELL> &q, double *scratch,
+template <int RPT, int BLK, class Sync, bool XREG, int SELL = 0>
+__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, QVec<RPT, (SELL != 0)> &q, double *scratch,
                                  const XVec<RPT, XREG> &xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
 {
     R.refresh();
@@ -175,13 +175,13 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, Q
 // The whole CG solve of one system by the workgroups that share `sy`.  Writes x to
 // a.x; every member returns the same CgResult.
 template <int RPT, int VAR, bool MIXED, int BLK, class Sync, int U = spmv_unroll<RPT, 2>(), bool XREG = false,
-          bool DYN = false, bool SELL = false>
+          bool DYN = false, int SELL = 0>
 __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q_lds,
                             double *xs_lds)
 {
     XVec<RPT, XREG> xs;
     xs.lds = xs_lds;
-    QVec<RPT, SELL> q;
+    QVec<RPT, (SELL != 0)> q;
     q.lds = q_lds;
     CgResult res;
     res.it = 0;
@@ -442,13 +442,13 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
 // those of the textbook recurrence.  A replacement (tile-mixed) publishes the corrected
 // p with a grid barrier and redoes the iteration.  Only the single-system kernel (all
 // workgroups of the launch) uses it.
-template <int RPT, bool MIXED, int BLK, int U, bool XREG, bool SELL>
+template <int RPT, bool MIXED, int BLK, int U, bool XREG, int SELL>
 __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, double *scratch, double *q_lds,
                                double *xs_lds)
 {
     XVec<RPT, XREG> xs;
     xs.lds = xs_lds;
-    QVec<RPT, SELL> q;
+    QVec<RPT, (SELL != 0)> q;
     q.lds = q_lds;
     CgResult res;
     res.it = 0;
@@ -603,8 +603,8 @@ __device__ __forceinline__ void publish_buf(const CgPart &a, Rows<RPT, BLK> &R, 
 }
 
 // q <- A v for the vector published in buffer `tb` (tile classes when MIXED).
-template <int RPT, bool MIXED, int BLK, int U, bool SELL>
-__device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int tb, QVec<RPT, SELL> &q,
+template <int RPT, bool MIXED, int BLK, int U, int SELL>
+__device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int tb, QVec<RPT, (SELL != 0)> &q,
                                          double *scratch)
 {
     spmv_op<MIXED, U, false, SELL>(a, R, tb ? a.p64b : a.p64, tb ? a.p32b : a.p32, q, scratch);
@@ -626,13 +626,13 @@ __device__ __forceinline__ void spmv_buf(const CgPart &a, Rows<RPT, BLK> &R, int
 // with binary32 products), and before accepting convergence, r = b - A x (binary64
 // stream), w = A r, s = A p and z = A s are recomputed from their definitions, keeping p.  Five row vectors live in registers,
 // so the launcher uses this variant only up to 2 rows per thread.
-template <int RPT, bool MIXED, int BLK, class Sync, int U, bool XREG, bool SELL = false>
+template <int RPT, bool MIXED, int BLK, class Sync, int U, bool XREG, int SELL = 0>
 __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double *scratch, double *q_lds,
                                       double *xs_lds)
 {
     XVec<RPT, XREG> xs;
     xs.lds = xs_lds;
-    QVec<RPT, SELL> q;
+    QVec<RPT, (SELL != 0)> q;
     q.lds = q_lds;
     CgResult res;
     res.it = 0;
@@ -810,7 +810,7 @@ finish:
     return res;
 }
 
-template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN, bool SELL = false>
+template <int RPT, int VAR, bool MIXED, int BLK, bool XREG, bool DYN, int SELL = 0>
 __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
 {
     extern __shared__ double smem[];

@@ -351,9 +351,12 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
     const DeviceLayout &L = A->L;
     const int R = P.G, NW = 1024 / 64, rpt = SL.rpt, n = L.n_rows;
     const size_t slots = (size_t)R * NW * rpt;
-    std::vector<int> nu[mfk::kSellStreams];
+    std::vector<int> Ls[mfk::kSellStreams], nu[mfk::kSellStreams];  // longest row, units per slot
     for (int k = 0; k < mfk::kSellStreams; k++)
-        if (mask >> k & 1) nu[k].assign(slots, 0);
+        if (mask >> k & 1) {
+            Ls[k].assign(slots, 0);
+            nu[k].assign(slots, 0);
+        }
     // visits the entries of the slice starting at row rs of range r: f(stream class, lane, entry, row, col)
     auto visit = [&](int r, int rs, auto &&f) {
         const int tr_end = P.h_tr[r + 1];
@@ -382,10 +385,31 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
                     if (!(mask >> st & 1)) continue;
                     int m = 0;
                     for (int l = 0; l < 64; l++) m = std::max(m, cnt[st][l]);
-                    const int E = mfk::sell_e_host(st);
-                    nu[st][slot] = (m + E - 1) / E;
+                    Ls[st][slot] = m;
                 }
             }
+    }
+    // unit profile of the class streams: narrow when it saves at least 15 % of their bytes
+    if (mask & 0xEu) {
+        double bw = 0, bn = 0;
+        for (int st = 1; st <= 3; st++) {
+            if (!(mask >> st & 1)) continue;
+            const int ew = mfk::sell_e_host(st, false), en = mfk::sell_e_host(st, true);
+            const double per = 64.0 * (2 + mfk::sell_vbytes_host(st));
+            for (size_t i = 0; i < slots; i++) {
+                bw += per * ew * ((Ls[st][i] + ew - 1) / ew);
+                bn += per * en * ((Ls[st][i] + en - 1) / en);
+            }
+        }
+        SL.narrow = bn < 0.85 * bw;
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int r = 0; r < R; r++) {
+        for (int st = 0; st < mfk::kSellStreams; st++) {
+            if (!(mask >> st & 1)) continue;
+            const int E = mfk::sell_e_host(st, SL.narrow);
+            for (size_t i = (size_t)r * NW * rpt; i < (size_t)(r + 1) * NW * rpt; i++) nu[st][i] = (Ls[st][i] + E - 1) / E;
+        }
         // an empty slice between two non-empty ones of a slot gets one zero unit (the
         // SpMV ends at most one slice per unit)
         for (int st = 0; st < mfk::kSellStreams; st++) {
@@ -416,7 +440,7 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
         soff[st][slots] = (int)u;
         if (u >= INT32_MAX / 64) return MF_ERR_TOO_LARGE;
         SL.units[st] = u;
-        const int E = mfk::sell_e_host(st);
+        const int E = mfk::sell_e_host(st, SL.narrow);
         idx[st].assign((size_t)u * 64 * E, 0);
         val[st].assign((size_t)u * 64 * E * mfk::sell_vbytes_host(st), 0);
     }
@@ -433,7 +457,7 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
                 const int base = rs - 1024 * k;
                 for (int st = 0; st < mfk::kSellStreams; st++) {
                     if (!(mask >> st & 1)) continue;
-                    const int E = mfk::sell_e_host(st);
+                    const int E = mfk::sell_e_host(st, SL.narrow);
                     for (int l = 0; l < 64; l++) {
                         const int16_t pad = (int16_t)(std::min(rs + l, n - 1) - (base + l));
                         for (int uu = soff[st][slot]; uu < soff[st][slot + 1]; uu++)
@@ -446,7 +470,7 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
                     const double x = L.h_ev[e];
                     for (int st : {c, (int)mfk::kSellAll64}) {
                         if (!(mask >> st & 1)) continue;
-                        const int E = mfk::sell_e_host(st);
+                        const int E = mfk::sell_e_host(st, SL.narrow);
                         const int p = pos[st][l]++;
                         const size_t i = ((size_t)(soff[st][slot] + p / E) * 64 + l) * E + p % E;
                         idx[st][i] = (int16_t)(col - (row - 1024 * k));
@@ -478,7 +502,8 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
     return st;
 }
 
-mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out)
+mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
+                      bool *narrow)
 {
     // the sliced-ELL streams are a cache of the partition, rebuilt with it
     Partition &P = const_cast<Partition &>(*Pc);
@@ -508,6 +533,7 @@ mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hi
         out->s[k].idx = SL.idx[k];
         out->s[k].val = SL.val[k];
     }
+    *narrow = SL.narrow;
     return MF_SUCCESS;
 }
 

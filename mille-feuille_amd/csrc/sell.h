@@ -38,11 +38,16 @@ struct DevSell {
     SellStream s[kSellStreams];
 };
 
-// entries per lane per unit, value bytes and arithmetic class of each stream
-template <int S> constexpr int sell_e() { return S == 0 || S == 4 ? 2 : S == 3 ? 8 : 4; }
+// Entries per lane per unit (E), value bytes and arithmetic class of each stream.  The
+// wide profile (E = 2/4/4/8 for binary64/32/16/int8) suits rows whose entries fall mostly
+// in one class; the narrow profile (2/2/2/4) pads less when rows spread over several
+// classes (each class stream pads every row of a slice to a multiple of its E).  The
+// layout builder picks the profile per matrix; kernels carry it as NAR.
+template <int S, bool NAR = false>
+constexpr int sell_e() { return S == 0 || S == 4 ? 2 : S == 3 ? (NAR ? 4 : 8) : (NAR ? 2 : 4); }
 template <int S> constexpr int sell_vbytes() { return S == 0 || S == 4 ? 8 : S == 1 ? 4 : S == 2 ? 2 : 1; }
 template <int S> constexpr int sell_cls() { return S == 4 ? 0 : S; }
-inline int sell_e_host(int s) { return s == 0 || s == 4 ? 2 : s == 3 ? 8 : 4; }
+inline int sell_e_host(int s, bool nar) { return s == 0 || s == 4 ? 2 : s == 3 ? (nar ? 4 : 8) : (nar ? 2 : 4); }
 inline int sell_vbytes_host(int s) { return s == 0 || s == 4 ? 8 : s == 1 ? 4 : s == 2 ? 2 : 1; }
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
@@ -85,9 +90,9 @@ __device__ __forceinline__ Words<NW> load_words(const void *base, size_t i)
 }
 
 // Pipeline registers of one unit: column words, value words, gathered x.
-template <int S>
+template <int S, bool NAR>
 struct SellUnit {
-    static constexpr int E = sell_e<S>();
+    static constexpr int E = sell_e<S, NAR>();
     static constexpr int NWI = E / 2;                       // 2 B per column offset
     static constexpr int NWV = E * sell_vbytes<S>() / 4;
     using XT = std::conditional_t<sell_cls<S>() == 0, double, float>;
@@ -96,20 +101,20 @@ struct SellUnit {
     XT xg[E];
 };
 
-template <int S>
-__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S> &U, int u, int lane)
+template <int S, bool NAR>
+__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR> &U, int u, int lane)
 {
     const size_t i = (size_t)u * 64 + lane;
-    U.d = load_words<SellUnit<S>::NWI>(st.idx, i);
-    U.v = load_words<SellUnit<S>::NWV>(st.val, i);
+    U.d = load_words<SellUnit<S, NAR>::NWI>(st.idx, i);
+    U.v = load_words<SellUnit<S, NAR>::NWV>(st.val, i);
 }
 
-template <int S>
-__device__ __forceinline__ void sell_gather(SellUnit<S> &U, int row, const double *__restrict__ x64,
+template <int S, bool NAR>
+__device__ __forceinline__ void sell_gather(SellUnit<S, NAR> &U, int row, const double *__restrict__ x64,
                                             const float *__restrict__ x32)
 {
 #pragma unroll
-    for (int j = 0; j < SellUnit<S>::E; j++) {
+    for (int j = 0; j < SellUnit<S, NAR>::E; j++) {
         const int col = row + (int)(short)(U.d.w[j >> 1] >> (16 * (j & 1)));
         if constexpr (sell_cls<S>() == 0) U.xg[j] = gptr<double>(x64)[col];
         else U.xg[j] = gptr<float>(x32)[col];
@@ -118,12 +123,12 @@ __device__ __forceinline__ void sell_gather(SellUnit<S> &U, int row, const doubl
 
 // Sum of the unit's products: binary64 for class 0, binary32 products added in binary64
 // otherwise (the tile classes' arithmetic of the chunk SpMV).
-template <int S>
-__device__ __forceinline__ double sell_products(const SellUnit<S> &U)
+template <int S, bool NAR>
+__device__ __forceinline__ double sell_products(const SellUnit<S, NAR> &U)
 {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < SellUnit<S>::E; j++) {
+    for (int j = 0; j < SellUnit<S, NAR>::E; j++) {
         if constexpr (sell_cls<S>() == 0) {
             s += __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]) * U.xg[j];
         } else if constexpr (S == 1) {
@@ -153,7 +158,7 @@ __device__ __forceinline__ double sell_products(const SellUnit<S> &U)
 // two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
 // and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
 // q's order.  Unit bounds are wave-uniform (scalar loads).
-template <int S, int RPT, int BLK>
+template <int S, int RPT, int BLK, bool NAR>
 __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
                                             const float *__restrict__ x32, double (&q)[RPT])
 {
@@ -178,21 +183,21 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
         rot++;
     };
     while (rot < RPT - 1 && e[0] <= u0) close_slice();  // leading empty slices
-    SellUnit<S> S0, S1, S2;
+    SellUnit<S, NAR> S0, S1, S2;
     double acc = 0.0;
     int i = u0;
-    sell_load<S>(st, S0, i, lane);
-    sell_load<S>(st, S1, min(i + 1, last), lane);
-    sell_gather<S>(S0, rowb, x64, x32);
+    sell_load<S, NAR>(st, S0, i, lane);
+    sell_load<S, NAR>(st, S1, min(i + 1, last), lane);
+    sell_gather<S, NAR>(S0, rowb, x64, x32);
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
     // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
-    auto step = [&](SellUnit<S> &a, SellUnit<S> &nb1, SellUnit<S> &nb2) {
-        sell_load<S>(st, nb2, min(i + 2, last), lane);
+    auto step = [&](SellUnit<S, NAR> &a, SellUnit<S, NAR> &nb1, SellUnit<S, NAR> &nb2) {
+        sell_load<S, NAR>(st, nb2, min(i + 2, last), lane);
         __builtin_amdgcn_sched_barrier(0);
-        sell_gather<S>(nb1, rowb, x64, x32);
+        sell_gather<S, NAR>(nb1, rowb, x64, x32);
         __builtin_amdgcn_sched_barrier(0);
-        const double pr = sell_products<S>(a);
+        const double pr = sell_products<S, NAR>(a);
         acc += i <= last ? pr : 0.0;
         const bool cl = i + 1 == e[0];  // unit i ends the current slice
         const double q0 = q[0] + acc;
@@ -251,7 +256,7 @@ struct RowVec {
 
 // q[k] = (A x) of this thread's rows.  MIXED: streams 0..3 (tile classes, binary32 x for
 // classes 1..3); otherwise stream 4 (binary64 values and x).
-template <bool MIXED, int RPT, int BLK>
+template <bool MIXED, int RPT, int BLK, bool NAR = false>
 __device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int row0, const double *__restrict__ x64,
                                           const float *__restrict__ x32, double (&q)[RPT])
 {
@@ -259,12 +264,12 @@ __device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int row0, 
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {
-        sell_stream<0, RPT, BLK>(D.s[0], slot, rowb, x64, x32, q);
-        sell_stream<1, RPT, BLK>(D.s[1], slot, rowb, x64, x32, q);
-        sell_stream<2, RPT, BLK>(D.s[2], slot, rowb, x64, x32, q);
-        sell_stream<3, RPT, BLK>(D.s[3], slot, rowb, x64, x32, q);
+        sell_stream<0, RPT, BLK, NAR>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK, NAR>(D.s[3], slot, rowb, x64, x32, q);
     } else {
-        sell_stream<4, RPT, BLK>(D.s[4], slot, rowb, x64, x32, q);
+        sell_stream<4, RPT, BLK, NAR>(D.s[4], slot, rowb, x64, x32, q);
     }
 }
 

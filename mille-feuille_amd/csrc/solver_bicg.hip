@@ -60,12 +60,12 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
 
 // q <- A v (tile classes when MIXED) for the vector published in x64/x32: sliced-ELL
 // streams into registers, or tile chunks into the workgroup's LDS slice.
-template <bool MIXED, int U, bool SELL, int RPT, int BLK>
+template <bool MIXED, int U, int SELL, int RPT, int BLK>
 __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr, int slot, Rows<RPT, BLK> &R,
-                                          const double *x64, const float *x32, QVec<RPT, SELL> &q, double *scratch)
+                                          const double *x64, const float *x32, QVec<RPT, (SELL != 0)> &q, double *scratch)
 {
-    if constexpr (SELL) {
-        spmv_sell<MIXED, RPT, BLK>(a.S, slot, R.row0, x64, x32, q.reg);
+    if constexpr (SELL != 0) {
+        spmv_sell<MIXED, RPT, BLK, SELL == 2>(a.S, slot, R.row0, x64, x32, q.reg);
     } else {
         zero_q<RPT, BLK>(q.lds, R);
         __syncthreads();
@@ -75,9 +75,9 @@ __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr
 }
 
 // r <- b - A x (binary64 stream); returns grid {r'r, rhat'r}.
-template <int RPT, int BLK, bool SELL>
+template <int RPT, int BLK, int SELL>
 __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int slot, int wg,
-                                   QVec<RPT, SELL> &q, double *xs, double *scratch, unsigned &epoch,
+                                   QVec<RPT, (SELL != 0)> &q, double *xs, double *scratch, unsigned &epoch,
                                    double (&r)[RPT], double &rr, double &rho)
 {
     R.refresh();
@@ -103,7 +103,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
     return true;
 }
 
-template <int RPT, int MIXED, int BLK, bool SELL = false>
+template <int RPT, int MIXED, int BLK, int SELL = 0>
 __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
 {
     extern __shared__ double smem[];
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     // SELL: SpMV results in registers, then x and v = A p in LDS after the reduction
     // scratch (v leaves the registers to the SpMV pipeline); otherwise rows_cap doubles
     // of SpMV result, then x, with v in registers
-    QVec<RPT, SELL> q;
+    QVec<RPT, (SELL != 0)> q;
     q.lds = SELL ? nullptr : smem + 64;
     double *xs = SELL ? smem + 64 : smem + 64 + a.A.rows_cap;
     double *vbuf = smem + 64 + a.A.rows_cap;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     unsigned epoch = 0;
 
     double r[RPT], p[RPT], sv[RPT];
-    RowVec<RPT, !SELL> v;
+    RowVec<RPT, SELL == 0> v;
     v.lds = vbuf;
     if (a.use_x0) bicg_spmv<false, U, SELL>(a, cr, slot, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         r[k] = p[k] = sv[k] = 0.0;
-        if constexpr (!SELL) v.reg[k] = 0.0;
+        if constexpr (SELL == 0) v.reg[k] = 0.0;
         if (!R.ok(k)) continue;
         const int row = R.row0 + R.loc(k);
         const double bk = a.b[row];
@@ -295,7 +295,7 @@ namespace mf {
 
 using BicgKernel = void (*)(mfk::BicgArgs);
 
-template <int MIXED, bool SELL>
+template <int MIXED, int SELL>
 static BicgKernel pick_bicg(int rpt)
 {
     switch (rpt) {
@@ -325,13 +325,14 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     const bool sell = want_sell && sell_supported(A, rpt);
     const size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    BicgKernel k = sell ? (mixed ? pick_bicg<1, true>(rpt) : pick_bicg<0, true>(rpt))
-                        : (mixed ? pick_bicg<1, false>(rpt) : pick_bicg<0, false>(rpt));
-    if (!k) return MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
     mfk::DevSell dsell;
     memset(&dsell, 0, sizeof(dsell));
-    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell)) != MF_SUCCESS) return st;
+    bool narrow = false;
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow)) != MF_SUCCESS) return st;
+    BicgKernel k = sell ? (mixed ? (narrow ? pick_bicg<1, 2>(rpt) : pick_bicg<1, 1>(rpt)) : pick_bicg<0, 1>(rpt))
+                        : (mixed ? pick_bicg<1, 0>(rpt) : pick_bicg<0, 0>(rpt));
+    if (!k) return MF_ERR_TOO_LARGE;
     mfk::BicgArgs a;
     a.A = A->L.dev(*P);
     a.gs.recs = A->ws.recs;
@@ -371,6 +372,7 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
         info->grid = G;
         info->block = B;
         info->rows_per_thread = rpt;
+        info->layout = sell ? (mixed && narrow ? 3 : 2) : 1;
         const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
         info->bytes_per_iter = 2.0 * M + 32.0 * A->L.n_rows;
         info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;

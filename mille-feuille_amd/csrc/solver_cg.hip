@@ -67,7 +67,7 @@ constexpr int kAutoMaxDeps = 64;
 
 // Rows per thread and block size: 1024 threads, up to 8 rows per thread (r and p in
 // at most 128 VGPRs per wave, 16 waves per CU).
-template <int VAR, bool MIXED, bool XREG, bool DYN, bool SELL = false>
+template <int VAR, bool MIXED, bool XREG, bool DYN, int SELL = 0>
 static CgKernel pick_cg(int rpt)
 {
     switch (rpt) {
@@ -81,7 +81,7 @@ static CgKernel pick_cg(int rpt)
 
 // Pipelined variant: five register vectors, instantiated up to 2 rows per thread
 // (more would spill).
-template <bool MIXED, bool XREG, bool SELL = false>
+template <bool MIXED, bool XREG, int SELL = 0>
 static CgKernel pick_cg3(int rpt)
 {
     switch (rpt) {
@@ -91,13 +91,25 @@ static CgKernel pick_cg3(int rpt)
     }
 }
 
-// Sliced-ELL kernels (static classes, x in LDS).
-static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed)
+// Sliced-ELL kernels (static classes, x in LDS); NAR: narrow unit profile of the class
+// streams.
+template <int NAR>
+static CgKernel cg_sell_mixed_for(int rpt, int var)
 {
-    if (var == 1) return mixed ? pick_cg<1, true, false, false, true>(rpt) : pick_cg<1, false, false, false, true>(rpt);
-    if (var == 3) return mixed ? pick_cg3<true, false, true>(rpt) : pick_cg3<false, false, true>(rpt);
-    if (var == 4) return mixed ? pick_cg<4, true, false, false, true>(rpt) : pick_cg<4, false, false, false, true>(rpt);
-    return mixed ? pick_cg<2, true, false, false, true>(rpt) : pick_cg<2, false, false, false, true>(rpt);
+    constexpr int SL = NAR ? 2 : 1;
+    if (var == 1) return pick_cg<1, true, false, false, SL>(rpt);
+    if (var == 3) return pick_cg3<true, false, SL>(rpt);
+    if (var == 4) return pick_cg<4, true, false, false, SL>(rpt);
+    return pick_cg<2, true, false, false, SL>(rpt);
+}
+
+static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed, bool narrow)
+{
+    if (mixed) return narrow ? cg_sell_mixed_for<1>(rpt, var) : cg_sell_mixed_for<0>(rpt, var);
+    if (var == 1) return pick_cg<1, false, false, false, 1>(rpt);
+    if (var == 3) return pick_cg3<false, false, 1>(rpt);
+    if (var == 4) return pick_cg<4, false, false, false, 1>(rpt);
+    return pick_cg<2, false, false, false, 1>(rpt);
 }
 
 // Variants: 1 textbook (3 syncs per iteration), 2 fused reductions (2 syncs),
@@ -182,12 +194,14 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
         lds = lds_x;
     }
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed) : cg_kernel_for(rpt, var, mixed, xreg, dyn);
-    if (!k) return var == 3 ? MF_ERR_UNSUPPORTED : MF_ERR_TOO_LARGE;
+    if (var == 3 && rpt > 2) return MF_ERR_UNSUPPORTED;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
     mfk::DevSell dsell;
     memset(&dsell, 0, sizeof(dsell));
-    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell)) != MF_SUCCESS) return st;
+    bool narrow = false;
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow)) != MF_SUCCESS) return st;
+    CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed, narrow) : cg_kernel_for(rpt, var, mixed, xreg, dyn);
+    if (!k) return var == 3 ? MF_ERR_UNSUPPORTED : MF_ERR_TOO_LARGE;
 
     mfk::CgArgs a;
     a.A = A->L.dev(*P);
@@ -234,6 +248,8 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
         info->grid = G;
         info->block = B;
         info->rows_per_thread = rpt;
+        info->layout = sell ? (mixed && narrow ? 3 : 2) : 1;
+        info->variant = var;
         const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
         info->bytes_per_iter = M + 16.0 * A->L.n_rows;
         info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;

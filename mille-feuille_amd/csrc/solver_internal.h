@@ -22,6 +22,7 @@ namespace mf {
 struct SellLayout {
     int rpt = 0;
     bool mixed = false;             // streams 0..3 built (stream 4 always)
+    bool narrow = false;            // unit profile of streams 1..3 (sell.h)
     int64_t units[mfk::kSellStreams] = {0, 0, 0, 0, 0};
     int *soff[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *idx[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -83,8 +84,9 @@ struct DeviceLayout {
 // streams: every column within the 16-bit offset range of its lane's slice-0 row.
 bool sell_supported(const mf_matrix *A, int rpt);
 // Sliced-ELL streams of partition P for rpt rows per thread (built and cached on first
-// use; `mixed` also builds the per-class streams).
-mf_status ensure_sell(mf_matrix *A, const Partition *P, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out);
+// use; `mixed` also builds the per-class streams); *narrow receives their unit profile.
+mf_status ensure_sell(mf_matrix *A, const Partition *P, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
+                      bool *narrow);
 
 // Per-matrix solver scratch (vectors, barrier records, status words).
 struct Workspace {

@@ -231,6 +231,7 @@ void fill_info(mf_solve_info *info, const MkCtx &c, int status, int it, int repl
     info->replacements = repl;
     info->grid = c.G;
     info->block = 1024;
+    info->layout = 1;  // tile chunks
     info->relres = bb > 0 ? std::sqrt(std::max(rr, 0.0) / bb) : 0.0;
     info->relres_true = bb > 0 ? std::sqrt(std::max(rt, 0.0) / bb) : std::sqrt(std::max(rt, 0.0));
     info->time_ms = ms;

@@ -111,3 +111,28 @@ def test_wide_offsets_fall_back_to_chunks(mf, orc, torch):
         assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
         assert info["iterations"] <= 1.5 * ref["iterations"] + 5
     A.close()
+
+
+def test_unit_profiles(mf, orc, torch):
+    """Rows spread over two classes (convection-diffusion: binary16 and int8 tiles) take the
+    narrow unit profile, single-class rows (Poisson: int8) the wide one; both reach the
+    FP64 tolerance, and the layout and variant used are reported."""
+    rp, ci, va = mf.workloads.convdiff3d(16)
+    b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    st = A.stats()["tiles_by_class"]
+    assert st[2] > 0 and st[3] > 0
+    x, info = A.bicgstab(b, tol=1e-10, precision="mixed")
+    assert info["status"] == 0 and info["layout"] == 3, info
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    x, info = A.bicgstab(b, tol=1e-10, precision="mixed", layout="chunks")
+    assert info["status"] == 0 and info["layout"] == 1
+    A.close()
+    rp, ci, va = mf.workloads.poisson3d(16)
+    b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    x, info = A.cg(b, tol=1e-10, precision="mixed")
+    assert info["status"] == 0 and info["layout"] == 2 and info["variant"] == 4, info
+    x, info = A.cg(b, tol=1e-10, precision="fp64")
+    assert info["status"] == 0 and info["layout"] == 1 and info["variant"] == 4, info
+    A.close()
