@@ -215,7 +215,11 @@ typedef struct mf_solve_options {
                               2 = fused reductions (2 grid syncs/iteration);
                               3 = pipelined (1 grid sync/iteration);
                               4 = one grid reduction/iteration, p published with point-to-point
-                                  flags to the workgroups that gather it */
+                                  flags to the workgroups that gather it.
+                              BiCGSTAB: 4 or 0 (default, when no workgroup's columns span more
+                              than 64 others) publish s and p point to point with two grid
+                              reductions per iteration; 1 or 2 make each publish a grid
+                              barrier (four per iteration). */
     int use_x0;            /* x holds an initial guess (else x0 = 0) */
     double watchdog_ms;    /* grid-barrier wait bound (0 = default 10000 ms) */
     int transpose;         /* solve with A^T (same tile stream read transposed) */

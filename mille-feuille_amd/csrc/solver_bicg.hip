@@ -6,10 +6,15 @@
 //   v = A p; alpha = rho / rhat'v; s = r - alpha v; t = A s; omega = t's / t't;
 //   x += alpha p + omega s; r = s - omega t; rho' = rhat'r;
 //   beta = (rho' / rho)(alpha / omega); p = r + beta (p - omega v).
-// Four grid syncs per iteration: rhat'v; publish s; {t's, t't, rhat's, rhat't}
-// (rho' = rhat's - omega rhat't); publish p with the directly summed r'r for the
-// convergence test.  Layout as in the CG kernel: r, p, v, s in registers, x and the
-// SpMV result in LDS, rhat in HBM (read-only), p and s published for the gathers.
+// Per iteration two grid reductions -- {rhat'v, r'r} after v = A p (the directly summed
+// r'r of the current r for the convergence and replacement tests, taken after that SpMV,
+// whose product is kept: a replacement changes r and rho, not p) and {t's, t't, rhat's,
+// rhat't} after t = A s (rho' = rhat's - omega rhat't) -- and two publishes, s and p.
+// With point-to-point publishes (nb_on) a workgroup waits only for the flags of the
+// workgroups owning its columns; otherwise each publish is a grid barrier.  The
+// reduction after each SpMV keeps a publish from overwriting a vector another workgroup
+// still gathers.  Layout as in the CG kernel: r, p, s in registers, v in registers or
+// LDS, x in LDS, rhat in HBM (read-only), p and s published for the gathers.
 // Mixed precision: both SpMVs read the tiles at their storage class with binary32
 // products; residual replacement r = b - A x (binary64 stream) with rho recomputed.
 #include <hip/hip_runtime.h>
@@ -40,6 +45,8 @@ struct BicgArgs {
     double tol2, delta2;
     int maxit, fixed, use_x0, need64, need32, max_repl;
     DevSell S;  // sliced-ELL streams (kernels instantiated with SELL)
+    NbSync nb;  // publish flags and dependency lists (nb_on)
+    int nb_on;
 };
 
 enum { BST_OK = 0, BST_NOTCONV = 4, BST_BREAK = 5, BST_TIMEOUT = 6 };
@@ -167,104 +174,126 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         status = BST_OK;
         goto done;
     }
-    for (;;) {
-        if (!a.fixed) {
-            bool conv = rr <= a.tol2 * bb;
-            if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                if (!bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, r, rr, rho)) {
+    {
+        bool p_vis = true;  // p published behind a grid barrier
+        unsigned pub = 0;   // flag epoch of the last point-to-point publish
+        for (;;) {
+            // v = A p ; {rhat'v, r'r}
+            if (!p_vis && !nb_wait(a.gs, a.nb, wg, pub)) {
+                status = BST_TIMEOUT;
+                break;
+            }
+            p_vis = false;
+            bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.p64, a.p32, q, scratch);
+            double d2[2] = {0.0, 0.0};
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                const double vk = q.get(k, R.loc(k));
+                v.set(k, R.loc(k), vk);
+                d2[0] += a.rhat[R.row0 + R.loc(k)] * vk;
+                d2[1] += r[k] * r[k];
+            }
+            if (!grid_sum<2>(a.gs, d2, epoch, false, wg, scratch)) {
+                status = BST_TIMEOUT;
+                break;
+            }
+            rr = d2[1];
+            if (!a.fixed) {
+                bool conv = rr <= a.tol2 * bb;
+                if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
+                    if (!bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, r, rr, rho)) {
+                        status = BST_TIMEOUT;
+                        break;
+                    }
+                    repl++;
+                    rbase = rr;
+                    conv = rr <= a.tol2 * bb;
+                }
+                if (conv) {
+                    status = BST_OK;
+                    break;
+                }
+            }
+            if (it >= a.maxit) {
+                status = a.fixed ? BST_OK : BST_NOTCONV;
+                break;
+            }
+            if (!(d2[0] != 0.0) || !isfinite(d2[0])) {
+                status = BST_BREAK;
+                break;
+            }
+            const double alpha = rho / d2[0];
+#pragma unroll
+            for (int k = 0; k < RPT; k++) sv[k] = SELL ? (R.ok(k) ? r[k] - alpha * v.get(k, R.loc(k)) : 0.0)
+                                                       : r[k] - alpha * v.get(k, R.loc(k));
+            publish<RPT, BLK>(a, R, sv, a.s64, a.s32);
+            if (a.nb_on) {
+                nb_signal(a.nb, wg, ++pub);
+                if (!nb_wait(a.gs, a.nb, wg, pub)) {
                     status = BST_TIMEOUT;
                     break;
                 }
-                repl++;
-                rbase = rr;
-                conv = rr <= a.tol2 * bb;
+            } else {
+                double z[1] = {0.0};
+                if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
+                    status = BST_TIMEOUT;
+                    break;
+                }
             }
-            if (conv) {
-                status = BST_OK;
+            // t = A s ; omega = t's / t't ; rho' = rhat's - omega rhat't
+            bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.s64, a.s32, q, scratch);
+            double d4[4] = {0.0, 0.0, 0.0, 0.0};
+            R.refresh();
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                const double tk = q.get(k, R.loc(k));
+                const double rh = a.rhat[R.row0 + R.loc(k)];
+                d4[0] += tk * sv[k];
+                d4[1] += tk * tk;
+                d4[2] += rh * sv[k];
+                d4[3] += rh * tk;
+            }
+            if (!grid_sum<4>(a.gs, d4, epoch, false, wg, scratch)) {
+                status = BST_TIMEOUT;
                 break;
             }
-        }
-        if (it >= a.maxit) {
-            status = a.fixed ? BST_OK : BST_NOTCONV;
-            break;
-        }
-        // v = A p ; alpha = rho / rhat'v
-        bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.p64, a.p32, q, scratch);
-        double d1[1] = {0.0};
-        R.refresh();
+            if (!(d4[1] != 0.0) || !isfinite(d4[1])) {
+                status = BST_BREAK;
+                break;
+            }
+            const double omega = d4[0] / d4[1];
+            const double rho_new = d4[2] - omega * d4[3];
+            if (!(omega != 0.0) || !(rho != 0.0) || !isfinite(rho_new)) {
+                status = BST_BREAK;
+                break;
+            }
+            const double beta = (rho_new / rho) * (alpha / omega);
+            R.refresh();
 #pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            if (!R.ok(k)) continue;
-            const double vk = q.get(k, R.loc(k));
-            v.set(k, R.loc(k), vk);
-            d1[0] += a.rhat[R.row0 + R.loc(k)] * vk;
+            for (int k = 0; k < RPT; k++) {
+                if (!R.ok(k)) continue;
+                const double tk = q.get(k, R.loc(k));
+                xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
+                r[k] = sv[k] - omega * tk;
+                p[k] = r[k] + beta * (p[k] - omega * v.get(k, R.loc(k)));
+            }
+            publish<RPT, BLK>(a, R, p, a.p64, a.p32);
+            if (a.nb_on) {
+                nb_signal(a.nb, wg, ++pub);
+            } else {
+                double z[1] = {0.0};
+                if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
+                    status = BST_TIMEOUT;
+                    break;
+                }
+                p_vis = true;
+            }
+            rho = rho_new;
+            it++;
         }
-        if (!grid_sum<1>(a.gs, d1, epoch, false, wg, scratch)) {
-            status = BST_TIMEOUT;
-            break;
-        }
-        if (!(d1[0] != 0.0) || !isfinite(d1[0])) {
-            status = BST_BREAK;
-            break;
-        }
-        const double alpha = rho / d1[0];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) sv[k] = SELL ? (R.ok(k) ? r[k] - alpha * v.get(k, R.loc(k)) : 0.0)
-                                                   : r[k] - alpha * v.get(k, R.loc(k));
-        publish<RPT, BLK>(a, R, sv, a.s64, a.s32);
-        double z[1] = {0.0};
-        if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
-            status = BST_TIMEOUT;
-            break;
-        }
-        // t = A s ; omega = t's / t't ; rho' = rhat's - omega rhat't
-        bicg_spmv<MIXED, U, SELL>(a, cr, slot, R, a.s64, a.s32, q, scratch);
-        double d4[4] = {0.0, 0.0, 0.0, 0.0};
-        R.refresh();
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            if (!R.ok(k)) continue;
-            const double tk = q.get(k, R.loc(k));
-            const double rh = a.rhat[R.row0 + R.loc(k)];
-            d4[0] += tk * sv[k];
-            d4[1] += tk * tk;
-            d4[2] += rh * sv[k];
-            d4[3] += rh * tk;
-        }
-        if (!grid_sum<4>(a.gs, d4, epoch, false, wg, scratch)) {
-            status = BST_TIMEOUT;
-            break;
-        }
-        if (!(d4[1] != 0.0) || !isfinite(d4[1])) {
-            status = BST_BREAK;
-            break;
-        }
-        const double omega = d4[0] / d4[1];
-        const double rho_new = d4[2] - omega * d4[3];
-        if (!(omega != 0.0) || !(rho != 0.0) || !isfinite(rho_new)) {
-            status = BST_BREAK;
-            break;
-        }
-        const double beta = (rho_new / rho) * (alpha / omega);
-        double e[1] = {0.0};
-        R.refresh();
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            if (!R.ok(k)) continue;
-            const double tk = q.get(k, R.loc(k));
-            xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
-            r[k] = sv[k] - omega * tk;
-            p[k] = r[k] + beta * (p[k] - omega * v.get(k, R.loc(k)));
-            e[0] += r[k] * r[k];
-        }
-        publish<RPT, BLK>(a, R, p, a.p64, a.p32);
-        if (!grid_sum<1>(a.gs, e, epoch, true, wg, scratch)) {
-            status = BST_TIMEOUT;
-            break;
-        }
-        rr = e[0];
-        rho = rho_new;
-        it++;
     }
 done:
     double rt = -1.0;
@@ -357,6 +386,12 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
     a.S = dsell;
+    a.nb.flags = A->ws.flags;
+    a.nb.dep_ptr = P->dep_ptr;
+    a.nb.dep_idx = P->dep_idx;
+    // point-to-point publishes unless asked for the grid-barrier form (variant 1 or 2) or a
+    // workgroup's columns span more than 64 others
+    a.nb_on = o->variant == 4 || (o->variant != 1 && o->variant != 2 && P->max_deps <= 64);
     float ms = 0;
     double h[32];
     if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;
@@ -373,6 +408,7 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
         info->block = B;
         info->rows_per_thread = rpt;
         info->layout = sell ? (mixed && narrow ? 3 : 2) : 1;
+        info->variant = a.nb_on ? 4 : 2;
         const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
         info->bytes_per_iter = 2.0 * M + 32.0 * A->L.n_rows;
         info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;

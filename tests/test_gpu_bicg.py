@@ -37,12 +37,13 @@ def _nonsym(mf, name):
     return rp, ci, va, b, xs
 
 
-@pytest.mark.parametrize("layout", ["auto", "sell"])
+@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("layout", ["auto", "chunks"])
 @pytest.mark.parametrize("name", ["convdiff12", "convdiff_ragged", "add20_like"])
-def test_bicgstab_fp64_iterations_and_residual(mf, orc, torch, name, layout):
+def test_bicgstab_fp64_iterations_and_residual(mf, orc, torch, name, layout, variant):
     rp, ci, va, b, xs = _nonsym(mf, name)
     A = mf.TiledMatrix(rp, ci, va)
-    x, info = A.bicgstab(b, tol=1e-10, layout=layout)
+    x, info = A.bicgstab(b, tol=1e-10, layout=layout, variant=variant)
     ref = orc.bicgstab(rp, ci, va, b, tol=1e-10)
     assert info["status"] == 0, info
     it, it_ref = info["iterations"], ref["iterations"]
@@ -52,23 +53,26 @@ def test_bicgstab_fp64_iterations_and_residual(mf, orc, torch, name, layout):
     A.close()
 
 
+@pytest.mark.parametrize("variant", [2, 4])
 @pytest.mark.parametrize("layout", ["auto", "chunks"])
 @pytest.mark.parametrize("name", ["convdiff12", "add20_like"])
-def test_bicgstab_mixed_reaches_fp64_tolerance(mf, orc, torch, name, layout):
+def test_bicgstab_mixed_reaches_fp64_tolerance(mf, orc, torch, name, layout, variant):
     rp, ci, va, b, xs = _nonsym(mf, name)
     A = mf.TiledMatrix(rp, ci, va, class_policy="rowscaled")
-    x, info = A.bicgstab(b, tol=1e-10, precision="mixed", layout=layout)
+    x, info = A.bicgstab(b, tol=1e-10, precision="mixed", layout=layout, variant=variant)
     assert info["status"] == 0, info
     rel = orc.true_relres(rp, ci, va, x.cpu().numpy(), b)
     assert rel <= 1.2e-10, (rel, info)
     A.close()
 
 
+@pytest.mark.parametrize("variant", [2, 4])
 @pytest.mark.parametrize("layout", ["chunks", "sell"])
-def test_bicgstab_fixed_iterations_match_oracle(mf, orc, torch, layout):
+def test_bicgstab_fixed_iterations_match_oracle(mf, orc, torch, layout, variant):
     rp, ci, va, b, xs = _nonsym(mf, "convdiff12")
     A = mf.TiledMatrix(rp, ci, va)
-    x, info = A.bicgstab(b, max_iter=6, fixed_iterations=True, layout=layout)
+    x, info = A.bicgstab(b, max_iter=6, fixed_iterations=True, layout=layout, variant=variant, grid=5)
+    assert info["variant"] == variant
     ref = orc.bicgstab(rp, ci, va, b, tol=0.0, maxit=6)
     assert info["iterations"] == 6
     assert np.allclose(x.cpu().numpy(), ref["x"], rtol=1e-9, atol=1e-12)
