@@ -244,8 +244,13 @@ typedef struct mf_solve_info {
     double bytes_per_iter;  /* algorithmic bytes per iteration (M + 16 n; BiCGSTAB 2M + 32 n) */
     double gbs;             /* bytes_per_iter * iterations / time */
     int layout;             /* SpMV layout used: 1 tile chunks, 2 sliced-ELL, 3 sliced-ELL with the
-                               narrow unit profile; variant of the CG recurrence in `variant` */
+                               narrow unit profile; 4 and 5: 2 and 3 with each workgroup's rows
+                               sorted by length.  The recurrence variant is in `variant`. */
     int variant;
+    int precision;          /* arithmetic used: MF_PREC_MIXED, or MF_PREC_FP64 -- also for a
+                               tile-mixed request when the matrix has no lower-precision tiles,
+                               or (layout default) when its class streams would move as many
+                               bytes as the binary64 stream */
 } mf_solve_info;
 
 void mf_solve_options_default(mf_solve_options *o);

@@ -95,7 +95,7 @@ class SolveInfo(ctypes.Structure):
         ("grid", ctypes.c_int), ("block", ctypes.c_int), ("rows_per_thread", ctypes.c_int),
         ("relres", ctypes.c_double), ("relres_true", ctypes.c_double), ("time_ms", ctypes.c_double),
         ("bytes_per_iter", ctypes.c_double), ("gbs", ctypes.c_double), ("layout", ctypes.c_int),
-        ("variant", ctypes.c_int),
+        ("variant", ctypes.c_int), ("precision", ctypes.c_int),
     ]
 
     def as_dict(self):

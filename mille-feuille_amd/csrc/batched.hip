@@ -506,6 +506,8 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
             in.grid = B->G;
             in.block = 1024;
             in.rows_per_thread = rpt;
+            in.layout = 1;  // tile chunks
+            in.precision = mixed ? MF_PREC_MIXED : MF_PREC_FP64;
         }
     }
     return (mf_status)worst;

@@ -102,7 +102,7 @@ __device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, cons
                                         QVec<RPT, (SELL != 0)> &qv, double *scratch)
 {
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, SELL == 2>(a.S, a.slot, R.row0, x64, x32, qv.reg);
+        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4)>(a.S, a.slot, R.base(), x64, x32, qv.reg);
     } else {
         zero_q<RPT, BLK>(qv.lds, R);
         __syncthreads();
@@ -123,7 +123,7 @@ __device__ __forceinline__ void publish_p(const CgPart &a, Rows<RPT, BLK> &R, co
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         if (a.need64) store_wt(&a.p64[row], p[k]);
         if (a.need32) store_wt(&a.p32[row], (float)p[k]);
     }
@@ -152,7 +152,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, Q
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs.get(k, R.loc(k)));
+        if (R.ok(k)) store_wt(&a.x[R.grow(k)], xs.get(k, R.loc(k)));
     double dummy[1] = {0.0};
     if (!sy.template sum<1>(dummy, true, scratch)) return false;
     spmv_op<false, spmv_unroll<RPT, XREG ? 3 : 2>(), false, SELL>(a, R, a.x, nullptr, q, scratch);
@@ -161,7 +161,7 @@ __device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, Q
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const double rt = a.b[R.row0 + R.loc(k)] - q.get(k, R.loc(k));
+        const double rt = a.b[R.grow(k)] - q.get(k, R.loc(k));
         if (fix_p) p[k] += rt - r[k];
         r[k] = rt;
         part[0] += rt * rt;
@@ -198,7 +198,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
     for (int k = 0; k < RPT; k++) {
         r[k] = p[k] = 0.0;
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         const double bk = a.b[row];
         xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
         r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
@@ -421,7 +421,7 @@ __device__ CgResult cg_core(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, double
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+            if (R.ok(k)) a.x[R.grow(k)] = xs.get(k, R.loc(k));
     }
     res.rr = rr;
     res.bb = bb;
@@ -464,7 +464,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     for (int k = 0; k < RPT; k++) {
         r[k] = p[k] = 0.0;
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         const double bk = a.b[row];
         xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
         r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
@@ -577,7 +577,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+            if (R.ok(k)) a.x[R.grow(k)] = xs.get(k, R.loc(k));
     }
     res.rr = rr;
     res.bb = bb;
@@ -596,7 +596,7 @@ __device__ __forceinline__ void publish_buf(const CgPart &a, Rows<RPT, BLK> &R, 
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         if (need64) store_wt(&d64[row], v[k]);
         if (need32) store_wt(&d32[row], (float)v[k]);
     }
@@ -652,7 +652,7 @@ __device__ CgResult cg_core_pipelined(const CgPart &a, Rows<RPT, BLK> &R, Sync &
     for (int k = 0; k < RPT; k++) {
         r[k] = w[k] = p[k] = sv[k] = z[k] = 0.0;
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         const double bk = a.b[row];
         xs.set(k, R.loc(k), a.use_x0 ? a.x[row] : 0.0);
         r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
@@ -803,7 +803,7 @@ finish:
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs.get(k, R.loc(k));
+            if (R.ok(k)) a.x[R.grow(k)] = xs.get(k, R.loc(k));
     }
     res.rr = rr;
     res.bb = bb;
@@ -825,6 +825,9 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     Rows<RPT, BLK> R;
     R.row0 = tr0 * kTile;
     R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
+    // SELL: 0 tile chunks; 1/2 sliced-ELL, wide/narrow unit profile; 3/4 the same with each
+    // range's rows permuted (sorted by length)
+    R.perm = SELL >= 3 ? a.S.perm + (size_t)wg * RPT * BLK : nullptr;
     R.refresh();
     CgPart part;
     part.A = a.A;

@@ -586,9 +586,18 @@ __device__ __forceinline__ int opaque(int v)
 template <int RPT, int BLK>
 struct Rows {
     int row0, nrows, t;
+    // local row of each thread slot t + k BLK (sliced-ELL streams built with a row
+    // permutation), or null for the identity
+    const int *perm = nullptr;
     __device__ __forceinline__ void refresh() { t = opaque((int)threadIdx.x); }
     __device__ __forceinline__ bool ok(int k) const { return t + k * BLK < nrows; }
     __device__ __forceinline__ int loc(int k) const { return t + k * BLK; }
+    // global row of slot k (valid when ok(k))
+    __device__ __forceinline__ int grow(int k) const { return row0 + (perm ? perm[loc(k)] : loc(k)); }
+    // the thread's slot-0 row, the base of the sliced-ELL column offsets: threads past the
+    // workgroup's rows use its last row (as the stream builder does), so every gather stays
+    // inside the matrix
+    __device__ __forceinline__ int base() const { return row0 + (perm ? perm[t] : min(t, nrows - 1)); }
 };
 
 template <int RPT, int BLK>

@@ -57,6 +57,7 @@ static void free_sell(SellLayout *SL)
         free_dev(SL->idx[k]);
         free_dev(SL->val[k]);
     }
+    free_dev(SL->perm);
     delete SL;
 }
 
@@ -334,52 +335,124 @@ mfk::DevMatrix DeviceLayout::dev(const Partition &P) const
 
 bool sell_supported(const mf_matrix *A, int rpt)
 {
-    return A->L.nnz > 0 && (int64_t)A->L.max_offset + 1024 * (int64_t)(rpt - 1) <= 32767 &&
+    // an offset from a slot-0 row: |col - row| plus at most 1024 rpt - 1 rows within the range
+    return A->L.nnz > 0 && (int64_t)A->L.max_offset + 1024 * (int64_t)rpt <= 32767 &&
            A->L.h_ev.size() == (size_t)A->L.nnz;
 }
 
+// Entries of each local row of range r (rows [r0, r1)), in tile order -- column tiles
+// ascending, so columns ascending within a row: ent[rp[lr] .. rp[lr+1]) are (entry, tile)
+// pairs.
+static void range_rows(const TileStruct &ts, int tr0, int tr1, int r0, int nr, std::vector<int> &rp,
+                       std::vector<std::pair<int64_t, int64_t>> &ent)
+{
+    rp.assign(nr + 1, 0);
+    for (int bi = tr0; bi < tr1; bi++)
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+            for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++) rp[bi * kTile + ts.lrow[e] - r0 + 1]++;
+    for (int i = 0; i < nr; i++) rp[i + 1] += rp[i];
+    ent.resize(rp[nr]);
+    std::vector<int> at(rp.begin(), rp.end() - 1);
+    for (int bi = tr0; bi < tr1; bi++)
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+            for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++)
+                ent[at[bi * kTile + ts.lrow[e] - r0]++] = {e, t};
+}
+
 // Builds the sliced-ELL streams in `mask` (bit s: stream s) for partition P with rpt
-// rows per thread of 1024-thread workgroups: slot (range r, wave v, slice k) covers rows
-// r0 + 1024 k + 64 v + [0, 64).  Pass 1 sizes every slot (longest row of the slice in
-// each stream, in units of E entries), pass 2 writes the entries of each row in tile
-// order -- column tiles ascending, so columns ascending -- at its lane, and the padding:
-// zero values at column offset 0, or at the last row for lanes past the matrix.
-static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &SL, unsigned mask, hipStream_t s)
+// rows per thread of 1024-thread workgroups.  Thread slot t + 1024 k of range r holds
+// local row perm[t + 1024 k]; slot (range r, wave v, slice k) covers slots
+// 1024 k + 64 v + [0, 64).
+//   * The permutation is fixed at the first build of the partition: the identity, or each
+//     range's rows stably sorted by length (longest first) when that cuts the padded
+//     binary64 stream by 10 % or more (rows of very different lengths in one slice pad to
+//     the longest).  Slots past the range's rows point at its last row.
+//   * Pass 1 sizes every slot (longest row of the slice in each stream); the unit profile
+//     of the class streams is then chosen (sell.h); an empty slice between two non-empty
+//     ones gets one zero unit.
+//   * Pass 2 writes each row's entries in tile order at its lane, as offsets from the
+//     lane's slot-0 row, and the padding: zero values at offset of the lane's own row (or
+//     of its slot-0 row for slots past the range).
+static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &SL, unsigned mask, bool allow_perm,
+                            hipStream_t s)
 {
     const TileStruct &ts = A->ts;
     const std::vector<uint8_t> &cls = A->cls;
     const DeviceLayout &L = A->L;
-    const int R = P.G, NW = 1024 / 64, rpt = SL.rpt, n = L.n_rows;
+    const int R = P.G, NW = 1024 / 64, rpt = SL.rpt, n = L.n_rows, LOC = 1024 * rpt;
     const size_t slots = (size_t)R * NW * rpt;
+    auto range_of = [&](int r, int &r0, int &nr) {
+        r0 = P.h_tr[r] * kTile;
+        nr = std::max(0, std::min(P.h_tr[r + 1] * kTile, n) - r0);
+    };
+    // ---- row permutation (first build of this partition) ----
+    if (SL.h_perm.empty()) {
+        SL.h_perm.assign((size_t)R * LOC, 0);
+        std::vector<double> pad_id(R, 0.0), pad_sorted(R, 0.0);
+        std::vector<std::vector<int>> sorted(R);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int r = 0; r < R; r++) {
+            int r0, nr;
+            range_of(r, r0, nr);
+            std::vector<int> len(nr, 0);
+            for (int bi = P.h_tr[r]; bi < P.h_tr[r + 1]; bi++)
+                for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+                    for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++) len[bi * kTile + ts.lrow[e] - r0]++;
+            std::vector<int> ord(nr);
+            for (int i = 0; i < nr; i++) ord[i] = i;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len[a] > len[b]; });
+            for (int s0 = 0; s0 < nr; s0 += 64) {
+                int mi = 0, ms = 0;
+                for (int l = s0; l < std::min(nr, s0 + 64); l++) {
+                    mi = std::max(mi, len[l]);
+                    ms = std::max(ms, len[ord[l]]);
+                }
+                pad_id[r] += (mi + 1) / 2 * 2;
+                pad_sorted[r] += (ms + 1) / 2 * 2;
+            }
+            sorted[r] = std::move(ord);
+        }
+        double pi = 0, ps = 0;
+        for (int r = 0; r < R; r++) {
+            pi += pad_id[r];
+            ps += pad_sorted[r];
+        }
+        SL.permuted = allow_perm && ps < 0.9 * pi;
+        for (int r = 0; r < R; r++) {
+            int r0, nr;
+            range_of(r, r0, nr);
+            int *pr = SL.h_perm.data() + (size_t)r * LOC;
+            for (int i = 0; i < LOC; i++)
+                pr[i] = i < nr ? (SL.permuted ? sorted[r][i] : i) : std::max(0, nr - 1);
+        }
+    }
     std::vector<int> Ls[mfk::kSellStreams], nu[mfk::kSellStreams];  // longest row, units per slot
     for (int k = 0; k < mfk::kSellStreams; k++)
         if (mask >> k & 1) {
             Ls[k].assign(slots, 0);
             nu[k].assign(slots, 0);
         }
-    // visits the entries of the slice starting at row rs of range r: f(stream class, lane, entry, row, col)
-    auto visit = [&](int r, int rs, auto &&f) {
-        const int tr_end = P.h_tr[r + 1];
-        for (int bi = rs / kTile; bi < std::min(rs / kTile + 4, tr_end); bi++)
-            for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
-                for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++) {
-                    const int row = bi * kTile + ts.lrow[e];
-                    f((int)cls[t], row - rs, e, row, ts.tile_col[t] * kTile + ts.lcol[e]);
-                }
-    };
 #pragma omp parallel for schedule(dynamic, 4)
     for (int r = 0; r < R; r++) {
-        const int r0 = P.h_tr[r] * kTile, r1 = std::min(P.h_tr[r + 1] * kTile, n);
+        int r0, nr;
+        range_of(r, r0, nr);
+        std::vector<int> rp;
+        std::vector<std::pair<int64_t, int64_t>> ent;
+        range_rows(ts, P.h_tr[r], P.h_tr[r + 1], r0, nr, rp, ent);
+        const int *pr = SL.h_perm.data() + (size_t)r * LOC;
         for (int v = 0; v < NW; v++)
             for (int k = 0; k < rpt; k++) {
-                const int rs = r0 + 1024 * k + 64 * v;
-                if (rs >= r1) continue;
+                const int l0 = 1024 * k + 64 * v;
+                if (l0 >= nr) continue;
                 int cnt[mfk::kSellStreams][64];
                 memset(cnt, 0, sizeof(cnt));
-                visit(r, rs, [&](int c, int l, int64_t, int, int) {
-                    cnt[c][l]++;
-                    cnt[mfk::kSellAll64][l]++;
-                });
+                for (int l = 0; l < 64 && l0 + l < nr; l++) {
+                    const int lr = pr[l0 + l];
+                    for (int i = rp[lr]; i < rp[lr + 1]; i++) {
+                        cnt[cls[ent[i].second]][l]++;
+                        cnt[mfk::kSellAll64][l]++;
+                    }
+                }
                 const size_t slot = ((size_t)r * NW + v) * rpt + k;
                 for (int st = 0; st < mfk::kSellStreams; st++) {
                     if (!(mask >> st & 1)) continue;
@@ -444,52 +517,69 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
         idx[st].assign((size_t)u * 64 * E, 0);
         val[st].assign((size_t)u * 64 * E * mfk::sell_vbytes_host(st), 0);
     }
-#pragma omp parallel for schedule(dynamic, 4)
+    bool range_ok = true;
+#pragma omp parallel for schedule(dynamic, 4) reduction(&& : range_ok)
     for (int r = 0; r < R; r++) {
-        const int r0 = P.h_tr[r] * kTile, r1 = std::min(P.h_tr[r + 1] * kTile, n);
+        int r0, nr;
+        range_of(r, r0, nr);
+        std::vector<int> rp;
+        std::vector<std::pair<int64_t, int64_t>> ent;
+        range_rows(ts, P.h_tr[r], P.h_tr[r + 1], r0, nr, rp, ent);
+        const int *pr = SL.h_perm.data() + (size_t)r * LOC;
         for (int v = 0; v < NW; v++)
             for (int k = 0; k < rpt; k++) {
-                const int rs = r0 + 1024 * k + 64 * v;
-                if (rs >= r1) continue;
+                const int l0 = 1024 * k + 64 * v;
+                if (l0 >= nr) continue;
                 const size_t slot = ((size_t)r * NW + v) * rpt + k;
-                // column words are offsets from the lane's slice-0 row rs - 1024 k + l;
-                // padding points at the lane's own row, or at the last row past the matrix
-                const int base = rs - 1024 * k;
+                int base[64], own[64];
+                for (int l = 0; l < 64; l++) {
+                    base[l] = r0 + pr[64 * v + l];
+                    own[l] = l0 + l < nr ? r0 + pr[l0 + l] : base[l];
+                }
                 for (int st = 0; st < mfk::kSellStreams; st++) {
                     if (!(mask >> st & 1)) continue;
                     const int E = mfk::sell_e_host(st, SL.narrow);
                     for (int l = 0; l < 64; l++) {
-                        const int16_t pad = (int16_t)(std::min(rs + l, n - 1) - (base + l));
+                        const int16_t pad = (int16_t)(own[l] - base[l]);
                         for (int uu = soff[st][slot]; uu < soff[st][slot + 1]; uu++)
                             for (int j = 0; j < E; j++) idx[st][((size_t)uu * 64 + l) * E + j] = pad;
                     }
                 }
                 int pos[mfk::kSellStreams][64];
                 memset(pos, 0, sizeof(pos));
-                visit(r, rs, [&](int c, int l, int64_t e, int row, int col) {
-                    const double x = L.h_ev[e];
-                    for (int st : {c, (int)mfk::kSellAll64}) {
-                        if (!(mask >> st & 1)) continue;
-                        const int E = mfk::sell_e_host(st, SL.narrow);
-                        const int p = pos[st][l]++;
-                        const size_t i = ((size_t)(soff[st][slot] + p / E) * 64 + l) * E + p % E;
-                        idx[st][i] = (int16_t)(col - (row - 1024 * k));
-                        uint8_t *d = val[st].data() + i * mfk::sell_vbytes_host(st);
-                        if (st == 0 || st == mfk::kSellAll64) {
-                            memcpy(d, &x, 8);
-                        } else if (st == 1) {
-                            const float f = (float)x;
-                            memcpy(d, &f, 4);
-                        } else if (st == 2) {
-                            const uint16_t h = f64_to_f16_rne(x);
-                            memcpy(d, &h, 2);
-                        } else {
-                            d[0] = (uint8_t)(int8_t)x;
+                for (int l = 0; l < 64 && l0 + l < nr; l++) {
+                    const int lr = pr[l0 + l];
+                    for (int i = rp[lr]; i < rp[lr + 1]; i++) {
+                        const int64_t e = ent[i].first, t = ent[i].second;
+                        const int c = cls[t];
+                        const int col = ts.tile_col[t] * kTile + ts.lcol[e];
+                        const int off = col - base[l];
+                        if (off < -32768 || off > 32767) range_ok = false;
+                        const double x = L.h_ev[e];
+                        for (int st : {c, (int)mfk::kSellAll64}) {
+                            if (!(mask >> st & 1)) continue;
+                            const int E = mfk::sell_e_host(st, SL.narrow);
+                            const int p = pos[st][l]++;
+                            const size_t q = ((size_t)(soff[st][slot] + p / E) * 64 + l) * E + p % E;
+                            idx[st][q] = (int16_t)off;
+                            uint8_t *d = val[st].data() + q * mfk::sell_vbytes_host(st);
+                            if (st == 0 || st == mfk::kSellAll64) {
+                                memcpy(d, &x, 8);
+                            } else if (st == 1) {
+                                const float f = (float)x;
+                                memcpy(d, &f, 4);
+                            } else if (st == 2) {
+                                const uint16_t h = f64_to_f16_rne(x);
+                                memcpy(d, &h, 2);
+                            } else {
+                                d[0] = (uint8_t)(int8_t)x;
+                            }
                         }
                     }
-                });
+                }
             }
     }
+    if (!range_ok) return MF_ERR_TOO_LARGE;
     mf_status st = MF_SUCCESS;
     for (int k = 0; k < mfk::kSellStreams && st == MF_SUCCESS; k++) {
         if (!(mask >> k & 1)) continue;
@@ -498,12 +588,26 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
         st = upload(&SL.val[k], val[k].data(), val[k].size(), s);
         SL.bytes += (double)sizeof(int) * soff[k].size() + 2.0 * idx[k].size() + (double)val[k].size();
     }
+    if (st == MF_SUCCESS && SL.permuted && !SL.perm)
+        st = upload((void **)&SL.perm, SL.h_perm.data(), sizeof(int) * SL.h_perm.size(), s);
     if (st == MF_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = MF_ERR_HIP;
     return st;
 }
 
+void sell_bytes(const Partition *P, double *class_streams, double *all64)
+{
+    *class_streams = *all64 = 0.0;
+    const SellLayout *SL = P->sell;
+    if (!SL) return;
+    for (int k = 0; k < mfk::kSellStreams; k++) {
+        const double b = (double)SL->units[k] * 64 * mfk::sell_e_host(k, SL->narrow) * (2 + mfk::sell_vbytes_host(k));
+        if (k == mfk::kSellAll64) *all64 += b;
+        else *class_streams += b;
+    }
+}
+
 mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
-                      bool *narrow)
+                      bool *narrow, bool allow_perm)
 {
     // the sliced-ELL streams are a cache of the partition, rebuilt with it
     Partition &P = const_cast<Partition &>(*Pc);
@@ -520,7 +624,7 @@ mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hi
     if (!SL.soff[mfk::kSellAll64]) mask |= 1u << mfk::kSellAll64;
     if (mixed && !SL.mixed) mask |= 0xFu;
     if (mask) {
-        mf_status st = build_sell(A, P, SL, mask, s);
+        mf_status st = build_sell(A, P, SL, mask, allow_perm, s);
         if (st != MF_SUCCESS) {
             free_sell(P.sell);
             P.sell = nullptr;
@@ -533,6 +637,7 @@ mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hi
         out->s[k].idx = SL.idx[k];
         out->s[k].val = SL.val[k];
     }
+    out->perm = SL.perm;
     *narrow = SL.narrow;
     return MF_SUCCESS;
 }

@@ -36,6 +36,10 @@ struct SellStream {
 
 struct DevSell {
     SellStream s[kSellStreams];
+    // per range, the local row of each thread slot t + 1024 k (rows sorted by length so
+    // that a slice holds rows of similar length), or null: slot t + 1024 k is local row
+    // t + 1024 k
+    const int *perm;
 };
 
 // Entries per lane per unit (E), value bytes and arithmetic class of each stream.  The
@@ -254,13 +258,13 @@ struct RowVec {
     }
 };
 
-// q[k] = (A x) of this thread's rows.  MIXED: streams 0..3 (tile classes, binary32 x for
-// classes 1..3); otherwise stream 4 (binary64 values and x).
+// q[k] = (A x) of this thread's rows; rowb is the thread's slot-0 row (the base of the
+// column offsets).  MIXED: streams 0..3 (tile classes, binary32 x for classes 1..3);
+// otherwise stream 4 (binary64 values and x).
 template <bool MIXED, int RPT, int BLK, bool NAR = false>
-__device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int row0, const double *__restrict__ x64,
+__device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                           const float *__restrict__ x32, double (&q)[RPT])
 {
-    const int rowb = row0 + (int)(threadIdx.x & ~63u) + (int)(threadIdx.x & 63);
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {

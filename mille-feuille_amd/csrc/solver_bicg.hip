@@ -59,7 +59,7 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         if (a.need64) store_wt(&d64[row], v[k]);
         if (a.need32) store_wt(&d32[row], (float)v[k]);
     }
@@ -72,7 +72,7 @@ __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr
                                           const double *x64, const float *x32, QVec<RPT, (SELL != 0)> &q, double *scratch)
 {
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, SELL == 2>(a.S, slot, R.row0, x64, x32, q.reg);
+        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4)>(a.S, slot, R.base(), x64, x32, q.reg);
     } else {
         zero_q<RPT, BLK>(q.lds, R);
         __syncthreads();
@@ -90,7 +90,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
     R.refresh();
 #pragma unroll
     for (int k = 0; k < RPT; k++)
-        if (R.ok(k)) store_wt(&a.x[R.row0 + R.loc(k)], xs[R.loc(k)]);
+        if (R.ok(k)) store_wt(&a.x[R.grow(k)], xs[R.loc(k)]);
     double dummy[1] = {0.0};
     if (!grid_sum<1>(a.gs, dummy, epoch, true, wg, scratch)) return false;
     bicg_spmv<false, spmv_unroll<RPT, 4>(), SELL>(a, cr, slot, R, a.x, nullptr, q, scratch);
@@ -99,7 +99,7 @@ __device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const l
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         r[k] = a.b[row] - q.get(k, R.loc(k));
         part[0] += r[k] * r[k];
         part[1] += a.rhat[row] * r[k];
@@ -128,6 +128,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
     Rows<RPT, BLK> R;
     R.row0 = tr0 * kTile;
     R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
+    R.perm = SELL >= 3 ? a.S.perm + (size_t)wg * RPT * BLK : nullptr;  // rows permuted (SELL 3/4)
     R.refresh();
     const long long *cr = a.A.wg_chunk + 4 * (size_t)wg;
     constexpr int U = spmv_unroll<RPT, 4>();
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         r[k] = p[k] = sv[k] = 0.0;
         if constexpr (SELL == 0) v.reg[k] = 0.0;
         if (!R.ok(k)) continue;
-        const int row = R.row0 + R.loc(k);
+        const int row = R.grow(k);
         const double bk = a.b[row];
         xs[R.loc(k)] = a.use_x0 ? a.x[row] : 0.0;
         r[k] = a.use_x0 ? bk - q.get(k, R.loc(k)) : bk;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
                 if (!R.ok(k)) continue;
                 const double vk = q.get(k, R.loc(k));
                 v.set(k, R.loc(k), vk);
-                d2[0] += a.rhat[R.row0 + R.loc(k)] * vk;
+                d2[0] += a.rhat[R.grow(k)] * vk;
                 d2[1] += r[k] * r[k];
             }
             if (!grid_sum<2>(a.gs, d2, epoch, false, wg, scratch)) {
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
             for (int k = 0; k < RPT; k++) {
                 if (!R.ok(k)) continue;
                 const double tk = q.get(k, R.loc(k));
-                const double rh = a.rhat[R.row0 + R.loc(k)];
+                const double rh = a.rhat[R.grow(k)];
                 d4[0] += tk * sv[k];
                 d4[1] += tk * tk;
                 d4[2] += rh * sv[k];
@@ -306,7 +307,7 @@ done:
         R.refresh();
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            if (R.ok(k)) a.x[R.row0 + R.loc(k)] = xs[R.loc(k)];
+            if (R.ok(k)) a.x[R.grow(k)] = xs[R.loc(k)];
     }
     if (wg == 0 && threadIdx.x == 0) {
         a.info[0] = (double)it;
@@ -336,6 +337,17 @@ static BicgKernel pick_bicg(int rpt)
     }
 }
 
+// rows permuted within each range (SELL 3/4): 4 and 8 rows per thread
+template <int MIXED, int SELL>
+static BicgKernel pick_bicg_perm(int rpt)
+{
+    switch (rpt) {
+    case 4: return mfk::bicg_kernel<4, MIXED, 1024, SELL>;
+    case 8: return mfk::bicg_kernel<8, MIXED, 1024, SELL>;
+    default: return nullptr;
+    }
+}
+
 mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *o, mf_solve_info *info,
                       hipStream_t s)
 {
@@ -351,16 +363,32 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     // sliced-ELL by default for both precisions (measured faster for binary64 too: its
     // kernel keeps v in LDS and spills far less than the tile-chunk kernel)
     const bool want_sell = o->layout != 1;
-    const bool sell = want_sell && sell_supported(A, rpt);
+    bool sell = want_sell && sell_supported(A, rpt);
     const size_t lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
     if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
     mfk::DevSell dsell;
     memset(&dsell, 0, sizeof(dsell));
     bool narrow = false;
-    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow)) != MF_SUCCESS) return st;
-    BicgKernel k = sell ? (mixed ? (narrow ? pick_bicg<1, 2>(rpt) : pick_bicg<1, 1>(rpt)) : pick_bicg<0, 1>(rpt))
-                        : (mixed ? pick_bicg<1, 0>(rpt) : pick_bicg<0, 0>(rpt));
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow, rpt >= 4)) != MF_SUCCESS) return st;
+    // as in the CG launcher: binary64 arithmetic when the class streams would not move fewer
+    // bytes than the binary64 stream
+    bool mixed_arith = mixed;
+    if (sell && mixed && o->layout == 0) {
+        double bc = 0, b64 = 0;
+        sell_bytes(P, &bc, &b64);
+        if (bc >= 0.95 * b64) mixed_arith = false;
+    }
+    BicgKernel k = nullptr;
+    if (sell && dsell.perm)
+        k = mixed_arith ? (narrow ? pick_bicg_perm<1, 4>(rpt) : pick_bicg_perm<1, 3>(rpt)) : pick_bicg_perm<0, 3>(rpt);
+    else if (sell)
+        k = mixed_arith ? (narrow ? pick_bicg<1, 2>(rpt) : pick_bicg<1, 1>(rpt)) : pick_bicg<0, 1>(rpt);
+    if (!k) {  // tile chunks
+        sell = false;
+        mixed_arith = mixed;
+        k = mixed ? pick_bicg<1, 0>(rpt) : pick_bicg<0, 0>(rpt);
+    }
     if (!k) return MF_ERR_TOO_LARGE;
     mfk::BicgArgs a;
     a.A = A->L.dev(*P);
@@ -382,8 +410,8 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     a.maxit = o->max_iter;
     a.fixed = o->fixed_iterations;
     a.use_x0 = o->use_x0;
-    a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
-    a.need32 = mixed ? 1 : 0;
+    a.need64 = (!mixed_arith || A->L.need_p64_mixed) ? 1 : 0;
+    a.need32 = mixed_arith ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
     a.S = dsell;
     a.nb.flags = A->ws.flags;
@@ -407,9 +435,10 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
         info->grid = G;
         info->block = B;
         info->rows_per_thread = rpt;
-        info->layout = sell ? (mixed && narrow ? 3 : 2) : 1;
+        info->layout = sell ? (mixed_arith && narrow ? 3 : 2) + (dsell.perm ? 2 : 0) : 1;
         info->variant = a.nb_on ? 4 : 2;
-        const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
+        info->precision = mixed_arith ? MF_PREC_MIXED : MF_PREC_FP64;
+        const double M = mixed_arith ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
         info->bytes_per_iter = 2.0 * M + 32.0 * A->L.n_rows;
         info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;
     }

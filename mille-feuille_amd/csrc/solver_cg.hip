@@ -103,8 +103,29 @@ static CgKernel cg_sell_mixed_for(int rpt, int var)
     return pick_cg<2, true, false, false, SL>(rpt);
 }
 
-static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed, bool narrow)
+// Kernels for rows permuted within each range (SELL 3/4): variants 2 and 4 at 4 and 8 rows
+// per thread, the shapes of the large irregular matrices the permutation is for.
+template <int VAR, bool MIXED, int SL>
+static CgKernel pick_cg_perm(int rpt)
 {
+    switch (rpt) {
+    case 4: return mfk::cg_kernel<4, VAR, MIXED, 1024, false, false, SL>;
+    case 8: return mfk::cg_kernel<8, VAR, MIXED, 1024, false, false, SL>;
+    default: return nullptr;
+    }
+}
+
+static bool cg_perm_supported(int rpt, int var) { return (var == 2 || var == 4) && (rpt == 4 || rpt == 8); }
+
+static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed, bool narrow, bool perm)
+{
+    if (perm) {
+        if (!cg_perm_supported(rpt, var)) return nullptr;
+        if (mixed)
+            return narrow ? (var == 4 ? pick_cg_perm<4, true, 4>(rpt) : pick_cg_perm<2, true, 4>(rpt))
+                          : (var == 4 ? pick_cg_perm<4, true, 3>(rpt) : pick_cg_perm<2, true, 3>(rpt));
+        return var == 4 ? pick_cg_perm<4, false, 3>(rpt) : pick_cg_perm<2, false, 3>(rpt);
+    }
     if (mixed) return narrow ? cg_sell_mixed_for<1>(rpt, var) : cg_sell_mixed_for<0>(rpt, var);
     if (var == 1) return pick_cg<1, false, false, false, 1>(rpt);
     if (var == 3) return pick_cg3<false, false, 1>(rpt);
@@ -183,10 +204,11 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     // registers.
     const size_t lds_x = (64 + (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     // Default: sliced-ELL for tile-mixed solves (its row-per-lane SpMV halves the mixed
-    // SpMV time); tile chunks for all-FP64 solves, whose streams are bandwidth-bound and
-    // where the chunks carry less padding.  layout 1 / 2 force chunks / sliced-ELL.
-    const bool want_sell = o->layout == 2 || (o->layout == 0 && mixed);
-    const bool sell = want_sell && !dyn && sell_supported(A, rpt) && lds_x <= (size_t)A->max_lds;
+    // SpMV time) and for matrices with few entries per tile; tile chunks for the other
+    // all-FP64 solves, whose streams are bandwidth-bound and where the chunks carry less
+    // padding.  layout 1 / 2 force chunks / sliced-ELL.
+    const bool want_sell = o->layout == 2 || (o->layout == 0 && (mixed || sparse_tiles(A)));
+    bool sell = want_sell && !dyn && sell_supported(A, rpt) && lds_x <= (size_t)A->max_lds;
     size_t lds = sell ? lds_x : (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     bool xreg = false;
     if (lds > (size_t)A->max_lds) {
@@ -199,8 +221,29 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     mfk::DevSell dsell;
     memset(&dsell, 0, sizeof(dsell));
     bool narrow = false;
-    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow)) != MF_SUCCESS) return st;
-    CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed, narrow) : cg_kernel_for(rpt, var, mixed, xreg, dyn);
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow, cg_perm_supported(rpt, var))) != MF_SUCCESS)
+        return st;
+    // Tile-mixed solve whose class streams would move about as many bytes as the binary64
+    // stream (rows spread over classes pad each class stream): binary64 arithmetic on the
+    // binary64 stream is then as fast and exact; reported in info->precision.
+    bool mixed_arith = mixed;
+    if (sell && mixed && o->layout == 0) {
+        double bc = 0, b64 = 0;
+        sell_bytes(P, &bc, &b64);
+        if (bc >= 0.95 * b64) mixed_arith = false;
+    }
+    CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed_arith, narrow, dsell.perm != nullptr) : nullptr;
+    if (sell && !k && dsell.perm) {
+        // the partition's streams were built permuted for another variant: tile chunks
+        sell = false;
+        mixed_arith = mixed;
+        lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
+        if (lds > (size_t)A->max_lds) {
+            xreg = true;
+            lds = lds_x;
+        }
+    }
+    if (!sell) k = cg_kernel_for(rpt, var, mixed, xreg, dyn);
     if (!k) return var == 3 ? MF_ERR_UNSUPPORTED : MF_ERR_TOO_LARGE;
 
     mfk::CgArgs a;
@@ -222,8 +265,8 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.maxit = o->max_iter;
     a.fixed = o->fixed_iterations;
     a.use_x0 = o->use_x0;
-    a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
-    a.need32 = mixed ? 1 : 0;
+    a.need64 = (!mixed_arith || A->L.need_p64_mixed) ? 1 : 0;
+    a.need32 = mixed_arith ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
     a.dvis = A->ws.dvis;
     a.dyn_mode = dyn ? (o->dynamic_precision == 2 ? 2 : 1) : 0;
@@ -248,9 +291,10 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
         info->grid = G;
         info->block = B;
         info->rows_per_thread = rpt;
-        info->layout = sell ? (mixed && narrow ? 3 : 2) : 1;
+        info->layout = sell ? (mixed_arith && narrow ? 3 : 2) + (dsell.perm ? 2 : 0) : 1;
         info->variant = var;
-        const double M = mixed ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
+        info->precision = mixed_arith ? MF_PREC_MIXED : MF_PREC_FP64;
+        const double M = mixed_arith ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;
         info->bytes_per_iter = M + 16.0 * A->L.n_rows;
         info->gbs = ms > 0 ? info->bytes_per_iter * info->iterations / (ms * 1e-3) / 1e9 : 0.0;
     }

@@ -23,6 +23,9 @@ struct SellLayout {
     int rpt = 0;
     bool mixed = false;             // streams 0..3 built (stream 4 always)
     bool narrow = false;            // unit profile of streams 1..3 (sell.h)
+    bool permuted = false;          // rows sorted by length within each range
+    std::vector<int> h_perm;        // host: local row of each thread slot, per range
+    int *perm = nullptr;            // device copy when permuted
     int64_t units[mfk::kSellStreams] = {0, 0, 0, 0, 0};
     int *soff[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *idx[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -81,12 +84,19 @@ struct DeviceLayout {
 };
 
 // Whether the persistent CG kernels with rpt rows per thread can use the sliced-ELL
-// streams: every column within the 16-bit offset range of its lane's slice-0 row.
+// streams: every column within the 16-bit offset range of its lane's slot-0 row (any row
+// of the range, the rows being permuted).
 bool sell_supported(const mf_matrix *A, int rpt);
 // Sliced-ELL streams of partition P for rpt rows per thread (built and cached on first
-// use; `mixed` also builds the per-class streams); *narrow receives their unit profile.
+// use; `mixed` also builds the per-class streams); *narrow receives their unit profile and
+// out->perm the row permutation (null: identity), which the first build may choose only
+// when allow_perm (the caller has kernels for it).
 mf_status ensure_sell(mf_matrix *A, const Partition *P, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
-                      bool *narrow);
+                      bool *narrow, bool allow_perm);
+// Padded bytes of one pass over the built class streams (0..3) and over the binary64
+// stream (4) of P's sliced-ELL layout.
+void sell_bytes(const Partition *P, double *class_streams, double *all64);
+
 
 // Per-matrix solver scratch (vectors, barrier records, status words).
 struct Workspace {
@@ -127,3 +137,9 @@ struct mf_matrix {
     std::map<long long, int> tuned_grid;  // autotuner cache: key -> workgroups
     int tuned_variant[2] = {0, 0};        // autotuner cache: fp64 / mixed CG variant (0 = default)
 };
+
+namespace mf {
+// Tile chunks carry a 12-byte header per chunk of up to 16 entries: with few entries per
+// tile they cost more per entry than the sliced-ELL column offsets.
+inline bool sparse_tiles(const mf_matrix *A) { return A->L.tiles > 0 && A->L.nnz < 8 * A->L.tiles; }
+}  // namespace mf

@@ -232,6 +232,7 @@ void fill_info(mf_solve_info *info, const MkCtx &c, int status, int it, int repl
     info->grid = c.G;
     info->block = 1024;
     info->layout = 1;  // tile chunks
+    info->precision = c.mixed ? MF_PREC_MIXED : MF_PREC_FP64;
     info->relres = bb > 0 ? std::sqrt(std::max(rr, 0.0) / bb) : 0.0;
     info->relres_true = bb > 0 ? std::sqrt(std::max(rt, 0.0) / bb) : std::sqrt(std::max(rt, 0.0));
     info->time_ms = ms;

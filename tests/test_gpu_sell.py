@@ -64,7 +64,30 @@ def test_sell_irregular_rows_and_classes(mf, orc, torch):
     xs_, xc_ = xs_.cpu().numpy(), xc_.cpu().numpy()
     assert np.max(np.abs(xs_ - xc_)) <= 1e-10 * np.abs(xc_).max()
     x, info = A.cg(b, tol=1e-10, precision="mixed", layout="sell")
-    assert info["status"] == 0
+    assert info["status"] == 0 and info["layout"] in (2, 3), info
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    A.close()
+
+
+@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sell_rows_sorted_by_length(mf, orc, torch, precision, variant):
+    """Four workgroups of 5,000 lognormal-length rows (8 rows per thread): each workgroup's
+    rows are sorted by length into slices (layout 4/5), against the oracle's iterate after
+    two iterations and to the FP64 tolerance; a variant without permuted kernels on the same
+    partition falls back to the tile chunks."""
+    rp, ci, va, b, xs = mf.workloads.irregular_spd(n=20000, band=400, logd_max=2.0)
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    ref = orc.cg(rp, ci, va, b, tol=0.0, maxit=2)
+    x, info = A.cg(b, max_iter=2, fixed_iterations=True, precision="fp64", layout="sell", grid=4,
+                   variant=variant)
+    assert info["layout"] in (4, 5) and info["rows_per_thread"] == 8, info
+    assert np.allclose(x.cpu().numpy(), ref["x"], rtol=1e-10, atol=1e-13 * np.abs(ref["x"]).max())
+    x, info = A.cg(b, tol=1e-10, precision=precision, layout="sell", grid=4, variant=variant)
+    assert info["status"] == 0 and info["layout"] in (4, 5), info
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    x, info = A.cg(b, tol=1e-10, precision=precision, layout="sell", grid=4, variant=1)
+    assert info["status"] == 0 and info["layout"] == 1, info
     assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
     A.close()
 
@@ -135,4 +158,27 @@ def test_unit_profiles(mf, orc, torch):
     assert info["status"] == 0 and info["layout"] == 2 and info["variant"] == 4, info
     x, info = A.cg(b, tol=1e-10, precision="fp64")
     assert info["status"] == 0 and info["layout"] == 1 and info["variant"] == 4, info
+    A.close()
+
+
+def test_layout_choice_by_bytes(mf, orc, torch):
+    """Few entries per tile: the all-FP64 CG takes the sliced-ELL streams (tile-chunk headers
+    would cost more per entry).  A tile-mixed request whose class streams would move as many
+    bytes as the binary64 stream runs binary64 arithmetic on it, reports so, and matches the
+    FP64 solve's iteration count."""
+    rp, ci, va, b, xs = mf.workloads.irregular_spd(n=20000, band=8000, logd_max=2.0)
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    st = A.stats()
+    assert ci.size < 8 * st["tiles"]
+    x, info = A.cg(b, tol=1e-10, precision="fp64")
+    assert info["status"] == 0 and info["layout"] in (2, 4), info
+    it64 = info["iterations"]
+    x, info = A.cg(b, tol=1e-10, precision="mixed")
+    assert info["status"] == 0
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    if info["precision"] == 0:
+        assert info["iterations"] == it64
+    # forcing the sliced-ELL layout keeps the tile classes
+    x, info = A.cg(b, tol=1e-10, precision="mixed", layout="sell")
+    assert info["status"] == 0 and info["precision"] == 1
     A.close()
