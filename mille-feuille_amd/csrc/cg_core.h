@@ -492,6 +492,11 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     }
     bool p_visible = true;  // p published behind a grid barrier (no flag wait needed)
     unsigned pub = 0;       // flag epoch of the last p published without one
+    // sliced-ELL: the first units of the next SpMV are loaded while the grid reduction,
+    // the update and the flag wait run (the matrix does not change)
+    constexpr bool NAR = SELL == 2 || SELL == 4;
+    SellLead<sell_lead_stream<MIXED>(), NAR> lead;
+    if constexpr (SELL != 0) sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
     while (alive && res.status == ST_NOTCONV) {
         if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
             res.status = ST_TIMEOUT;
@@ -499,7 +504,12 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         }
         p_visible = false;
         // q = A p
-        spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
+        if constexpr (SELL != 0) {
+            spmv_sell_lead<MIXED, RPT, BLK, NAR>(a.S, a.slot, R.base(), a.p64, a.p32, q.reg, lead);
+            sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
+        } else {
+            spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
+        }
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         R.refresh();
 #pragma unroll

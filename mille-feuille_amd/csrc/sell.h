@@ -162,9 +162,32 @@ __device__ __forceinline__ double sell_products(const SellUnit<S, NAR> &U)
 // two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
 // and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
 // q's order.  Unit bounds are wave-uniform (scalar loads).
-template <int S, int RPT, int BLK, bool NAR>
+// Column and value words of a stream's first two units for this wave, loaded ahead of the
+// SpMV that sums them (they do not depend on the vector): the SpMV then starts with only
+// the gathers to wait for.
+template <int S, bool NAR>
+struct SellLead {
+    SellUnit<S, NAR> u0, u1;
+};
+
+template <int S, int RPT, bool NAR>
+__device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, SellLead<S, NAR> &L)
+{
+    const int lane = threadIdx.x & 63;
+    const int *sb = st.soff + (size_t)slot * RPT;
+    const int u0 = __builtin_amdgcn_readfirstlane(sb[0]);
+    const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
+    if (u0 > last) return;  // the wave has no unit in this stream (the SpMV skips it too)
+    sell_load<S, NAR>(st, L.u0, u0, lane);
+    sell_load<S, NAR>(st, L.u1, min(u0 + 1, last), lane);
+}
+
+// LEAD: the first two units were loaded by sell_prefetch into *lead (whose registers the
+// pipeline then reuses: prefetch again before the next use).
+template <int S, int RPT, int BLK, bool NAR, bool LEAD = false>
 __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
-                                            const float *__restrict__ x32, double (&q)[RPT])
+                                            const float *__restrict__ x32, double (&q)[RPT],
+                                            SellLead<S, NAR> *lead = nullptr)
 {
     const int lane = threadIdx.x & 63;
     const int *sb = st.soff + (size_t)slot * RPT;
@@ -187,11 +210,15 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
         rot++;
     };
     while (rot < RPT - 1 && e[0] <= u0) close_slice();  // leading empty slices
-    SellUnit<S, NAR> S0, S1, S2;
+    SellUnit<S, NAR> l0, l1, S2;
+    SellUnit<S, NAR> &S0 = LEAD ? lead->u0 : l0;
+    SellUnit<S, NAR> &S1 = LEAD ? lead->u1 : l1;
     double acc = 0.0;
     int i = u0;
-    sell_load<S, NAR>(st, S0, i, lane);
-    sell_load<S, NAR>(st, S1, min(i + 1, last), lane);
+    if constexpr (!LEAD) {
+        sell_load<S, NAR>(st, S0, i, lane);
+        sell_load<S, NAR>(st, S1, min(i + 1, last), lane);
+    }
     sell_gather<S, NAR>(S0, rowb, x64, x32);
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
@@ -261,6 +288,27 @@ struct RowVec {
 // q[k] = (A x) of this thread's rows; rowb is the thread's slot-0 row (the base of the
 // column offsets).  MIXED: streams 0..3 (tile classes, binary32 x for classes 1..3);
 // otherwise stream 4 (binary64 values and x).
+// The stream whose first units the variant-4 CG kernels prefetch: int8 (the class the
+// mixed SpMV sums first) or binary64.
+template <bool MIXED> constexpr int sell_lead_stream() { return MIXED ? 3 : kSellAll64; }
+
+template <bool MIXED, int RPT, int BLK, bool NAR = false>
+__device__ __forceinline__ void spmv_sell_lead(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
+                                               const float *__restrict__ x32, double (&q)[RPT],
+                                               SellLead<sell_lead_stream<MIXED>(), NAR> &lead)
+{
+#pragma unroll
+    for (int k = 0; k < RPT; k++) q[k] = 0.0;
+    if constexpr (MIXED) {
+        sell_stream<3, RPT, BLK, NAR, true>(D.s[3], slot, rowb, x64, x32, q, &lead);
+        sell_stream<0, RPT, BLK, NAR>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR>(D.s[2], slot, rowb, x64, x32, q);
+    } else {
+        sell_stream<4, RPT, BLK, NAR, true>(D.s[4], slot, rowb, x64, x32, q, &lead);
+    }
+}
+
 template <bool MIXED, int RPT, int BLK, bool NAR = false>
 __device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                           const float *__restrict__ x32, double (&q)[RPT])
