@@ -146,7 +146,7 @@ __device__ __forceinline__ void classify_segments(const CgPart &a, Rows<RPT, BLK
 // r <- b - A x with the binary64 stream (x published to a.x first); p += (r_new - r_old)
 // when fix_p; returns the r'r sum.  One SpMV and two syncs.
 template <int RPT, int BLK, class Sync, bool XREG, int SELL = 0>
-__device__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, QVec<RPT, (SELL != 0)> &q, double *scratch,
+__device__ __forceinline__ bool replace_residual(const CgPart &a, Rows<RPT, BLK> &R, Sync &sy, QVec<RPT, (SELL != 0)> &q, double *scratch,
                                  const XVec<RPT, XREG> &xs, double (&r)[RPT], double (&p)[RPT], double &rr, bool fix_p)
 {
     R.refresh();

@@ -179,7 +179,7 @@ __device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned l
 // the broadcast tag, so every member leaves at the same sync without a further read of
 // the flag.  Returns false when the group aborts.
 template <int K>
-__device__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bool visibility, int wg,
+__device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bool visibility, int wg,
                          double *scratch)
 {
     __shared__ double s_part[kMaxRed * 16];  // per-wave partials of this member

@@ -83,7 +83,7 @@ __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr
 
 // r <- b - A x (binary64 stream); returns grid {r'r, rhat'r}.
 template <int RPT, int BLK, int SELL>
-__device__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int slot, int wg,
+__device__ __forceinline__ bool bicg_true_residual(const BicgArgs &a, Rows<RPT, BLK> &R, const long long *cr, int slot, int wg,
                                    QVec<RPT, (SELL != 0)> &q, double *xs, double *scratch, unsigned &epoch,
                                    double (&r)[RPT], double &rr, double &rho)
 {
@@ -275,11 +275,21 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
             R.refresh();
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
-                if (!R.ok(k)) continue;
-                const double tk = q.get(k, R.loc(k));
-                xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
-                r[k] = sv[k] - omega * tk;
-                p[k] = r[k] + beta * (p[k] - omega * v.get(k, R.loc(k)));
+                if constexpr (SELL != 0) {
+                    // every register row is assigned (zero past the range), so r is not
+                    // live across the SpMV t = A s of the next iteration
+                    const bool ok = R.ok(k);
+                    const double vk = ok ? v.get(k, R.loc(k)) : 0.0;
+                    if (ok) xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
+                    r[k] = ok ? sv[k] - omega * q.reg[k] : 0.0;
+                    p[k] = ok ? r[k] + beta * (p[k] - omega * vk) : 0.0;
+                } else {
+                    if (!R.ok(k)) continue;
+                    const double tk = q.get(k, R.loc(k));
+                    xs[R.loc(k)] += alpha * p[k] + omega * sv[k];
+                    r[k] = sv[k] - omega * tk;
+                    p[k] = r[k] + beta * (p[k] - omega * v.get(k, R.loc(k)));
+                }
             }
             publish<RPT, BLK>(a, R, p, a.p64, a.p32);
             if (a.nb_on) {
