@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +57,7 @@ static void free_sell(SellLayout *SL)
         free_dev(SL->soff[k]);
         free_dev(SL->idx[k]);
         free_dev(SL->val[k]);
+        free_dev(SL->lmask[k]);
     }
     free_dev(SL->perm);
     delete SL;
@@ -373,6 +375,14 @@ static void range_rows(const TileStruct &ts, int tr0, int tr1, int r0, int nr, s
 //   * Pass 2 writes each row's entries in tile order at its lane, as offsets from the
 //     lane's slot-0 row, and the padding: zero values at offset of the lane's own row (or
 //     of its slot-0 row for slots past the range).
+//   * Shared column words (identity row order): when the union of a slice's relative
+//     columns (col - row) in a stream fits the slice's units, every row -- and every slot
+//     past the range -- is laid out on that union in ascending order, with zero values
+//     where the row has no entry; then all lanes hold the same column words.  The slice
+//     keeps the plain layout when a row's columns in the stream are not strictly
+//     ascending or an aligned column of some lane falls outside [0, n).
+//   * lmask marks the waves whose column words agree in every lane of every unit (0, the
+//     kernels read lane 0's words) or not (63).
 static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &SL, unsigned mask, bool allow_perm,
                             hipStream_t s)
 {
@@ -536,13 +546,54 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
                     base[l] = r0 + pr[64 * v + l];
                     own[l] = l0 + l < nr ? r0 + pr[l0 + l] : base[l];
                 }
+                // union of the relative columns per stream (own - base is 1024 k in every
+                // lane of the identity order, so aligned rows share their column words)
+                std::vector<int> U[mfk::kSellStreams];
+                bool shared[mfk::kSellStreams] = {false, false, false, false, false};
+                if (!SL.permuted) {
+                    bool asc[mfk::kSellStreams] = {true, true, true, true, true};
+                    for (int l = 0; l < 64 && l0 + l < nr; l++) {
+                        const int lr = pr[l0 + l];
+                        int prev[mfk::kSellStreams] = {INT_MIN, INT_MIN, INT_MIN, INT_MIN, INT_MIN};
+                        for (int i = rp[lr]; i < rp[lr + 1]; i++) {
+                            const int64_t e = ent[i].first, t = ent[i].second;
+                            const int rel = ts.tile_col[t] * kTile + ts.lcol[e] - own[l];
+                            for (int st : {(int)cls[t], (int)mfk::kSellAll64}) {
+                                if (!(mask >> st & 1)) continue;
+                                if (rel <= prev[st]) asc[st] = false;
+                                prev[st] = rel;
+                                U[st].push_back(rel);
+                            }
+                        }
+                    }
+                    for (int st = 0; st < mfk::kSellStreams; st++) {
+                        if (!(mask >> st & 1)) continue;
+                        std::sort(U[st].begin(), U[st].end());
+                        U[st].erase(std::unique(U[st].begin(), U[st].end()), U[st].end());
+                        const int64_t cap =
+                            (int64_t)(soff[st][slot + 1] - soff[st][slot]) * mfk::sell_e_host(st, SL.narrow);
+                        shared[st] = asc[st] && (int64_t)U[st].size() <= cap;
+                        // every aligned column (zero entries and padding included) must be a
+                        // column of the matrix in every lane
+                        for (int l = 0; l < 64 && shared[st]; l++) {
+                            const int64_t c0 = (int64_t)base[l] + 1024 * k;
+                            const int64_t lo = c0 + (U[st].empty() ? 0 : std::min(U[st].front(), 0));
+                            const int64_t hi = c0 + (U[st].empty() ? 0 : std::max(U[st].back(), 0));
+                            shared[st] = lo >= 0 && hi < n;
+                        }
+                    }
+                }
                 for (int st = 0; st < mfk::kSellStreams; st++) {
                     if (!(mask >> st & 1)) continue;
                     const int E = mfk::sell_e_host(st, SL.narrow);
                     for (int l = 0; l < 64; l++) {
                         const int16_t pad = (int16_t)(own[l] - base[l]);
                         for (int uu = soff[st][slot]; uu < soff[st][slot + 1]; uu++)
-                            for (int j = 0; j < E; j++) idx[st][((size_t)uu * 64 + l) * E + j] = pad;
+                            for (int j = 0; j < E; j++) {
+                                const size_t pp = (size_t)(uu - soff[st][slot]) * E + j;
+                                idx[st][((size_t)uu * 64 + l) * E + j] =
+                                    shared[st] ? (int16_t)((pp < U[st].size() ? U[st][pp] : 0) + 1024 * k) : pad;
+                            }
                     }
                 }
                 int pos[mfk::kSellStreams][64];
@@ -559,7 +610,9 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
                         for (int st : {c, (int)mfk::kSellAll64}) {
                             if (!(mask >> st & 1)) continue;
                             const int E = mfk::sell_e_host(st, SL.narrow);
-                            const int p = pos[st][l]++;
+                            const int p = shared[st] ? (int)(std::lower_bound(U[st].begin(), U[st].end(),
+                                                                              col - own[l]) - U[st].begin())
+                                                     : pos[st][l]++;
                             const size_t q = ((size_t)(soff[st][slot] + p / E) * 64 + l) * E + p % E;
                             idx[st][q] = (int16_t)off;
                             uint8_t *d = val[st].data() + q * mfk::sell_vbytes_host(st);
@@ -580,9 +633,30 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
             }
     }
     if (!range_ok) return MF_ERR_TOO_LARGE;
+    // waves whose column words agree in every lane of every unit
+    std::vector<int> lm[mfk::kSellStreams];
+    for (int st = 0; st < mfk::kSellStreams; st++) {
+        if (!(mask >> st & 1)) continue;
+        lm[st].assign((size_t)R * NW, 63);
+        const int E = mfk::sell_e_host(st, SL.narrow);
+        int64_t su = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : su)
+        for (int w = 0; w < R * NW; w++) {
+            const int u0 = soff[st][(size_t)w * rpt], u1 = soff[st][(size_t)w * rpt + rpt];
+            bool same = true;
+            for (int u = u0; u < u1 && same; u++) {
+                const int16_t *wd = idx[st].data() + (size_t)u * 64 * E;
+                for (int j = E; j < 64 * E && same; j++) same = wd[j] == wd[j % E];
+            }
+            lm[st][w] = same ? 0 : 63;
+            su += same ? u1 - u0 : 0;
+        }
+        SL.shared_units[st] = su;
+    }
     mf_status st = MF_SUCCESS;
     for (int k = 0; k < mfk::kSellStreams && st == MF_SUCCESS; k++) {
         if (!(mask >> k & 1)) continue;
+        if ((st = upload((void **)&SL.lmask[k], lm[k].data(), sizeof(int) * lm[k].size(), s)) != MF_SUCCESS) break;
         if ((st = upload((void **)&SL.soff[k], soff[k].data(), sizeof(int) * soff[k].size(), s)) != MF_SUCCESS) break;
         if ((st = upload(&SL.idx[k], idx[k].data(), sizeof(int16_t) * idx[k].size(), s)) != MF_SUCCESS) break;
         st = upload(&SL.val[k], val[k].data(), val[k].size(), s);
@@ -636,6 +710,8 @@ mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hi
         out->s[k].soff = SL.soff[k];
         out->s[k].idx = SL.idx[k];
         out->s[k].val = SL.val[k];
+        out->s[k].lmask = SL.lmask[k];
+        out->s[k].xmax = A->L.n_rows - 1;
     }
     out->perm = SL.perm;
     *narrow = SL.narrow;

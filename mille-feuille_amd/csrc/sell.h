@@ -12,7 +12,13 @@
 //     i.e. whole cache lines;
 //   * column words are 16-bit offsets from the lane's slice-0 row (col - row + BLK k), so
 //     the index stream costs 2 B per entry and the gathers do not depend on the slice;
-//     matrices where that offset leaves the int16 range use the tile-chunk layout.
+//     matrices where that offset leaves the int16 range use the tile-chunk layout;
+//   * shared column words: where the rows of a slice have the same columns relative to
+//     their own row (stencils), the builder aligns every row to the slice's union of
+//     relative columns (zero values where a row lacks one), so all 64 lanes hold the same
+//     column words; a wave whose slices are all like that reads lane 0's words in every
+//     lane (one cache line per unit instead of 64 lanes' words).  The builder shares a
+//     slice's words only where every aligned column lies in [0, n).
 // Streams 0..3 hold the entries of tiles of storage class 0..3 (binary64, binary32,
 // binary16, int8 values) for the tile-mixed SpMV; stream 4 holds every entry in binary64
 // for the all-FP64 SpMV and the residual replacement.  A row's sum runs over stream 0,
@@ -32,6 +38,8 @@ struct SellStream {
     const int *soff;   // first unit of each (range, wave, slice) slot, slots in that order, + end
     const void *idx;   // [unit][64][E] int16 offsets of the column from the slice-0 row
     const void *val;   // [unit][64][E] values at the stream's width
+    const int *lmask;  // per (range, wave): 0 when every lane holds the same column words, else 63
+    int xmax;          // last column (n - 1)
 };
 
 struct DevSell {
@@ -56,6 +64,37 @@ inline int sell_vbytes_host(int s) { return s == 0 || s == 4 ? 8 : s == 1 ? 4 : 
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// Gathers of x go through buffer loads in index mode: the load takes the element index in
+// one VGPR (no 64-bit address arithmetic), and the hardware checks it against n (an index
+// outside [0, n) reads 0, never memory; the layouts only store columns inside).
+__device__ float sell_buf_load_f32(i32x4 rsrc, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.buffer.load.f32");
+__device__ double sell_buf_load_f64(i32x4 rsrc, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.buffer.load.f64");
+
+// Buffer resource of n elements of `stride` bytes at p (gfx9 layout: base address bits
+// 0..47, stride in word 1 bits 16..29, record count, word 3 = 32-bit data format).
+__device__ __forceinline__ i32x4 sell_rsrc(const void *p, int stride, int n)
+{
+    const unsigned long long a = (unsigned long long)p;
+    i32x4 r;
+    r.x = (int)(unsigned)a;
+    r.y = (int)(((unsigned)(a >> 32) & 0xffffu) | ((unsigned)stride << 16));
+    r.z = n;
+    r.w = 0x00020000;
+    return r;
+}
+
+// x as the gathers see it: binary64 and binary32 copies of n elements.
+struct SellX {
+    i32x4 r64, r32;
+};
+__device__ __forceinline__ SellX sell_x(const double *x64, const float *x32, int n)
+{
+    return SellX{sell_rsrc(x64, 8, n), sell_rsrc(x32, 4, n)};
+}
 
 template <int NW>
 struct Words {
@@ -105,23 +144,22 @@ struct SellUnit {
     XT xg[E];
 };
 
+// ilane: the lane whose column words this lane reads (lane & the wave's lmask)
 template <int S, bool NAR>
-__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR> &U, int u, int lane)
+__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR> &U, int u, int lane, int ilane)
 {
-    const size_t i = (size_t)u * 64 + lane;
-    U.d = load_words<SellUnit<S, NAR>::NWI>(st.idx, i);
-    U.v = load_words<SellUnit<S, NAR>::NWV>(st.val, i);
+    U.d = load_words<SellUnit<S, NAR>::NWI>(st.idx, (size_t)u * 64 + ilane);
+    U.v = load_words<SellUnit<S, NAR>::NWV>(st.val, (size_t)u * 64 + lane);
 }
 
 template <int S, bool NAR>
-__device__ __forceinline__ void sell_gather(SellUnit<S, NAR> &U, int row, const double *__restrict__ x64,
-                                            const float *__restrict__ x32)
+__device__ __forceinline__ void sell_gather(SellUnit<S, NAR> &U, int row, const SellX &X)
 {
 #pragma unroll
     for (int j = 0; j < SellUnit<S, NAR>::E; j++) {
         const int col = row + (int)(short)(U.d.w[j >> 1] >> (16 * (j & 1)));
-        if constexpr (sell_cls<S>() == 0) U.xg[j] = gptr<double>(x64)[col];
-        else U.xg[j] = gptr<float>(x32)[col];
+        if constexpr (sell_cls<S>() == 0) U.xg[j] = sell_buf_load_f64(X.r64, col, 0, 0, 0);
+        else U.xg[j] = sell_buf_load_f32(X.r32, col, 0, 0, 0);
     }
 }
 
@@ -147,21 +185,6 @@ __device__ __forceinline__ double sell_products(const SellUnit<S, NAR> &U)
     return s;
 }
 
-// q[k] += (A x)[row0 + BLK k + 64 v + lane] over stream S's entries.  Column words are
-// offsets from the lane's slice-0 row rowb = row0 + 64 v + lane, so the gathers do not
-// depend on the slice.
-// Three-stage software pipeline over the wave's units: unit i+2's column and value words
-// are loaded and unit i+1's x gathered while unit i is summed into acc.  Three register
-// sets rotate (the loop body is unrolled three times), so no register with a load in
-// flight is copied, and the loop has a single exit test per three steps and no other
-// branch, so the compiler keeps each load where it is issued (a branch between a load and
-// its use lets it sink the load next to the use and wait for it there).  Steps past the
-// last unit load the last unit again and add nothing.
-// The slice being summed is always q[0]: when unit i ends it, acc is added to q[0] and q
-// rotates down by one (selects, no branch).  The layout gives every empty slice between
-// two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
-// and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
-// q's order.  Unit bounds are wave-uniform (scalar loads).
 // Column and value words of a stream's first two units for this wave, loaded ahead of the
 // SpMV that sums them (they do not depend on the vector): the SpMV then starts with only
 // the gathers to wait for.
@@ -178,10 +201,26 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
     const int u0 = __builtin_amdgcn_readfirstlane(sb[0]);
     const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
     if (u0 > last) return;  // the wave has no unit in this stream (the SpMV skips it too)
-    sell_load<S, NAR>(st, L.u0, u0, lane);
-    sell_load<S, NAR>(st, L.u1, min(u0 + 1, last), lane);
+    const int ilane = lane & __builtin_amdgcn_readfirstlane(st.lmask[slot]);
+    sell_load<S, NAR>(st, L.u0, u0, lane, ilane);
+    sell_load<S, NAR>(st, L.u1, min(u0 + 1, last), lane, ilane);
 }
 
+// q[k] += (A x)[row0 + BLK k + 64 v + lane] over stream S's entries.  Column words are
+// offsets from the lane's slice-0 row rowb = row0 + 64 v + lane, so the gathers do not
+// depend on the slice.
+// Three-stage software pipeline over the wave's units: unit i+2's column and value words
+// are loaded and unit i+1's x gathered while unit i is summed into acc.  Three register
+// sets rotate (the loop body is unrolled three times), so no register with a load in
+// flight is copied, and the loop has a single exit test per three steps and no other
+// branch, so the compiler keeps each load where it is issued (a branch between a load and
+// its use lets it sink the load next to the use and wait for it there).  Steps past the
+// last unit load the last unit again and add nothing.
+// The slice being summed is always q[0]: when unit i ends it, acc is added to q[0] and q
+// rotates down by one (selects, no branch).  The layout gives every empty slice between
+// two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
+// and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
+// q's order.  Unit bounds are wave-uniform (scalar loads).
 // LEAD: the first two units were loaded by sell_prefetch into *lead (whose registers the
 // pipeline then reuses: prefetch again before the next use).
 template <int S, int RPT, int BLK, bool NAR, bool LEAD = false>
@@ -194,6 +233,8 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
     const int u0 = __builtin_amdgcn_readfirstlane(sb[0]);
     const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
     if (u0 > last) return;
+    const int ilane = lane & __builtin_amdgcn_readfirstlane(st.lmask[slot]);
+    const SellX X = sell_x(x64, x32, st.xmax + 1);
     // ends of the slices not yet closed, in scalar registers (e[0]: the current one)
     int e[RPT];
 #pragma unroll
@@ -216,17 +257,17 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
     double acc = 0.0;
     int i = u0;
     if constexpr (!LEAD) {
-        sell_load<S, NAR>(st, S0, i, lane);
-        sell_load<S, NAR>(st, S1, min(i + 1, last), lane);
+        sell_load<S, NAR>(st, S0, i, lane, ilane);
+        sell_load<S, NAR>(st, S1, min(i + 1, last), lane, ilane);
     }
-    sell_gather<S, NAR>(S0, rowb, x64, x32);
+    sell_gather<S, NAR>(S0, rowb, X);
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
     // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
     auto step = [&](SellUnit<S, NAR> &a, SellUnit<S, NAR> &nb1, SellUnit<S, NAR> &nb2) {
-        sell_load<S, NAR>(st, nb2, min(i + 2, last), lane);
+        sell_load<S, NAR>(st, nb2, min(i + 2, last), lane, ilane);
         __builtin_amdgcn_sched_barrier(0);
-        sell_gather<S, NAR>(nb1, rowb, x64, x32);
+        sell_gather<S, NAR>(nb1, rowb, X);
         __builtin_amdgcn_sched_barrier(0);
         const double pr = sell_products<S, NAR>(a);
         acc += i <= last ? pr : 0.0;

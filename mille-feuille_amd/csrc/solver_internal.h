@@ -30,6 +30,8 @@ struct SellLayout {
     int *soff[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *idx[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *val[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int *lmask[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per wave slot
+    int64_t shared_units[mfk::kSellStreams] = {0, 0, 0, 0, 0};  // units of shared-column waves
     double bytes = 0;               // device bytes of the built streams
 };
 
