@@ -38,8 +38,8 @@ def cpu_model():
 
 def run_extras(mf, torch, dev, log):
     """Batched CG over 512 SPD systems (one persistent hipGraph launch vs 512 single-system
-    launches), the differentiable solve on 64^3 variable-coefficient diffusion, and the
-    irregular SPD system at n = 10^6."""
+    launches), the differentiable solve on 64^3 variable-coefficient diffusion, the
+    irregular SPD system at n = 10^6, and plain FP64 CG on 2D Poisson 64 x 64."""
     out = {}
     systems = mf.workloads.batched_poisson(512, seed=3)
     B = mf.Batch([(rp, ci, va) for rp, ci, va, _ in systems], class_policy="lossless")
@@ -65,16 +65,21 @@ def run_extras(mf, torch, dev, log):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq_kernel = 0.0
+    bpi = []  # algorithmic bytes per CG iteration of each system (M + 16 n)
     for M, bb in zip(mats, bs):
         _, inf = M.cg(bb, tol=1e-10, precision="mixed")
         seq_kernel += inf["time_ms"]
+        bpi.append(inf["bytes_per_iter"])
     torch.cuda.synchronize()
     seq_wall = (time.perf_counter() - t0) * 1e3
     for M in mats:
         M.close()
     tb = float(np.median(t_b))
+    alg = sum(b_ * i_["iterations"] for b_, i_ in zip(bpi, infos))
     out["batched_512"] = {"systems": len(systems), "converged": conv, "worst_relres_true_host": worst,
                           "launch_ms": tb, "solves_per_s": len(systems) / (tb * 1e-3),
+                          "iterations_total": int(sum(i_["iterations"] for i_ in infos)),
+                          "alg_GBs": alg / (tb * 1e-3) / 1e9,
                           "sequential_kernel_ms": seq_kernel, "sequential_wall_ms": seq_wall,
                           "sequential_solves_per_s": len(systems) / (seq_wall * 1e-3),
                           "speedup_vs_sequential_wall": seq_wall / tb}
@@ -104,11 +109,26 @@ def run_extras(mf, torch, dev, log):
     fi, ai = op.last_info.get("forward", {}), op.last_info.get("adjoint", {})
     out["autograd_64cubed"] = {"n": n, "nnz": int(ci.size), "forward_backward_ms": float(np.median(times)),
                                "forward_iterations": fi.get("iterations"), "adjoint_iterations": ai.get("iterations"),
-                               "forward_kernel_ms": fi.get("time_ms"), "adjoint_kernel_ms": ai.get("time_ms")}
+                               "forward_kernel_ms": fi.get("time_ms"), "adjoint_kernel_ms": ai.get("time_ms"),
+                               "forward_alg_GBs": fi.get("gbs"), "adjoint_alg_GBs": ai.get("gbs")}
     log(f"[bench] autograd 64^3: forward+backward {np.median(times):.2f} ms "
         f"(it {fi.get('iterations')} + {ai.get('iterations')})")
     op.close()
     out["irregular_spd_1m"] = run_irregular(mf, torch, dev, log)
+    # the small CPU-comparable case: plain FP64 CG on 2D 5-point Poisson 64 x 64 (x* seed 0)
+    rp, ci, va = mf.workloads.poisson2d(64)
+    b_np, _ = mf.workloads.rhs_from_solution(rp, ci, va, 0)
+    A = mf.TiledMatrix(rp, ci, va)
+    bt = torch.as_tensor(b_np, device=dev)
+    A.cg(bt, tol=1e-10, precision="fp64")
+    runs = [A.cg(bt, tol=1e-10, precision="fp64")[1] for _ in range(5)]
+    info = sorted(runs, key=lambda i: i["time_ms"])[2]
+    out["poisson2d_64_fp64"] = {"n": int(rp.size - 1), "nnz": int(ci.size), "iterations": info["iterations"],
+                                "kernel_ms": info["time_ms"], "relres_true": info["relres_true"],
+                                "alg_GBs": info["gbs"], "grid": info["grid"]}
+    log(f"[bench] 2D Poisson 64^2 FP64 CG: {info['time_ms']:.3f} ms, {info['iterations']} iterations, "
+        f"relres {info['relres_true']:.2e}")
+    A.close()
     return out
 
 

@@ -196,6 +196,21 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
     __syncthreads();
     epoch++;
     const int G = gs.G;
+    if (G == 1) {
+        // the workgroup is the grid: its barrier is the grid barrier (each wave waited for
+        // its write-through stores before it), and an L1 invalidate makes them visible
+        if (threadIdx.x < K) {
+            double m = 0.0;
+            for (int i = 0; i < nwv; i++) m += s_part[threadIdx.x * 16 + i];
+            s_bc[threadIdx.x] = m;
+        }
+        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = s_bc[k];
+        __syncthreads();
+        return true;
+    }
     const long long t0 = __builtin_readcyclecounter();
     // wave 0 combines the wave partials in a fixed order and publishes this member's
     // record; with visibility every wave has waited for its write-through stores before
