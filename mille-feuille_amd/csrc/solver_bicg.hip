@@ -16,7 +16,12 @@
 // still gathers.  Layout as in the CG kernel: r, p, s in registers, v in registers or
 // LDS, x in LDS, rhat in HBM (read-only), p and s published for the gathers.
 // Mixed precision: both SpMVs read the tiles at their storage class with binary32
-// products; residual replacement r = b - A x (binary64 stream) with rho recomputed.
+// products; residual replacement r = b - A x (binary64 stream) with rho recomputed.  A
+// breakdown (rhat'v, t't, omega or rho zero) or a residual grown 100-fold over the last
+// replacement's restarts the recurrence from that binary64 residual with rhat = p = r,
+// within the replacement budget; the solve stops with a breakdown when the binary64
+// residual has run a million-fold above the smallest one (x diverged) or the budget is
+// spent.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -176,9 +181,54 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
         goto done;
     }
     {
-        bool p_vis = true;  // p published behind a grid barrier
-        unsigned pub = 0;   // flag epoch of the last point-to-point publish
+        bool p_vis = true;     // p published behind a grid barrier
+        unsigned pub = 0;      // flag epoch of the last point-to-point publish
+        int fresh = 0;         // tile-mixed: 1 replace the residual, 2 restart the recurrence
+        double rbest = rbase;  // smallest binary64 residual r'r seen (replacements, restarts)
         for (;;) {
+            if (MIXED && fresh) {
+                // r = b - A x in binary64 (rho = rhat'r).  A restart also takes a new shadow
+                // vector rhat = r and p = r, rho = r'r: the recurrence starts again from the
+                // current x (binary32 products can drive rho or omega to zero near
+                // stagnation, or make the recurrence grow).  The SpMV v = A p follows again.
+                if (!bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, r, rr, rho)) {
+                    status = BST_TIMEOUT;
+                    break;
+                }
+                repl++;
+                rbase = rr;
+                if (!(rr <= 1e12 * rbest)) {  // x itself has run away: stop
+                    status = BST_BREAK;
+                    break;
+                }
+                rbest = fmin(rbest, rr);
+                if (fresh == 2) {
+                    rho = rr;
+                    R.refresh();
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) {
+                        const bool ok = R.ok(k);
+                        if (ok) a.rhat[R.grow(k)] = r[k];
+                        p[k] = ok ? r[k] : 0.0;
+                    }
+                    publish<RPT, BLK>(a, R, p, a.p64, a.p32);
+                    double z[1] = {0.0};
+                    if (!grid_sum<1>(a.gs, z, epoch, true, wg, scratch)) {
+                        status = BST_TIMEOUT;
+                        break;
+                    }
+                    p_vis = true;
+                }
+                fresh = 0;
+                if (!a.fixed && rr <= a.tol2 * bb) {
+                    status = BST_OK;
+                    break;
+                }
+                if (!(rr > 0.0)) {
+                    status = rr == 0.0 ? BST_OK : BST_BREAK;
+                    break;
+                }
+            }
             // v = A p ; {rhat'v, r'r}
             if (!p_vis && !nb_wait(a.gs, a.nb, wg, pub)) {
                 status = BST_TIMEOUT;
@@ -201,16 +251,23 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
                 break;
             }
             rr = d2[1];
+            // tile-mixed: a residual grown 100-fold (in norm) over the last replacement's
+            // means the binary32 products have broken the recurrence; restart it, or stop
+            // once the replacement budget is spent
+            if (MIXED && !a.fixed && !(rr <= 1e4 * rbase)) {
+                if (repl < a.max_repl) {
+                    fresh = 2;
+                    continue;
+                }
+                status = BST_BREAK;
+                break;
+            }
             if (!a.fixed) {
-                bool conv = rr <= a.tol2 * bb;
+                const bool conv = rr <= a.tol2 * bb;
                 if (MIXED && repl < a.max_repl && (conv || rr < a.delta2 * rbase)) {
-                    if (!bicg_true_residual<RPT, BLK, SELL>(a, R, cr, slot, wg, q, xs, scratch, epoch, r, rr, rho)) {
-                        status = BST_TIMEOUT;
-                        break;
-                    }
-                    repl++;
-                    rbase = rr;
-                    conv = rr <= a.tol2 * bb;
+                    // replace the residual and redo the iteration from v = A p
+                    fresh = 1;
+                    continue;
                 }
                 if (conv) {
                     status = BST_OK;
@@ -222,6 +279,10 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
                 break;
             }
             if (!(d2[0] != 0.0) || !isfinite(d2[0])) {
+                if (MIXED && repl < a.max_repl) {
+                    fresh = 2;
+                    continue;
+                }
                 status = BST_BREAK;
                 break;
             }
@@ -262,12 +323,20 @@ __global__ __launch_bounds__(BLK) void bicg_kernel(BicgArgs a)
                 break;
             }
             if (!(d4[1] != 0.0) || !isfinite(d4[1])) {
+                if (MIXED && repl < a.max_repl) {
+                    fresh = 2;
+                    continue;
+                }
                 status = BST_BREAK;
                 break;
             }
             const double omega = d4[0] / d4[1];
             const double rho_new = d4[2] - omega * d4[3];
             if (!(omega != 0.0) || !(rho != 0.0) || !isfinite(rho_new)) {
+                if (MIXED && repl < a.max_repl) {
+                    fresh = 2;
+                    continue;
+                }
                 status = BST_BREAK;
                 break;
             }

@@ -66,6 +66,22 @@ def test_bicgstab_mixed_reaches_fp64_tolerance(mf, orc, torch, name, layout, var
     A.close()
 
 
+@pytest.mark.parametrize("layout", ["auto", "chunks"])
+def test_bicgstab_mixed_restarts_after_breakdown(mf, orc, torch, layout):
+    """2D Poisson 256 x 256 (int8 tiles): with binary32 products the textbook recurrence
+    reaches a zero rho or omega before 1e-10; the tile-mixed solve restarts from the binary64
+    residual (rhat = p = r) and still reaches the FP64 tolerance."""
+    rp, ci, va = mf.workloads.poisson2d(256)
+    b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    print(f"bicg restart poisson2d 256 {layout}", flush=True)
+    x, info = A.bicgstab(b, tol=1e-10, precision="mixed", layout=layout, max_iter=20000)
+    print(info, flush=True)
+    assert info["status"] == 0 and info["replacements"] >= 1, info
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    A.close()
+
+
 @pytest.mark.parametrize("variant", [2, 4])
 @pytest.mark.parametrize("layout", ["chunks", "sell"])
 def test_bicgstab_fixed_iterations_match_oracle(mf, orc, torch, layout, variant):
