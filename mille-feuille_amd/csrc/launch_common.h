@@ -36,15 +36,16 @@ inline long long watchdog_cycles(double ms)
     return (long long)(ms * 1e-3 * 2.5e9);  // s_memtime ticks at most at the shader clock
 }
 
-int default_grid(const mf_matrix *A, int maxG);
+int default_grid(const mf_matrix *A, int maxG, bool mixed_cg = false);
 
 // Pick the grid for a persistent kernel of `kind` (autotuned value if present).
-inline int choose_grid(mf_matrix *A, const mf_solve_options *o, long long key)
+// mixed_cg: the grid of a tile-mixed CG solve (sliced-ELL, row sums in registers).
+inline int choose_grid(mf_matrix *A, const mf_solve_options *o, long long key, bool mixed_cg = false)
 {
     if (o->grid > 0) return o->grid;
     auto it = A->tuned_grid.find(key);
     if (it != A->tuned_grid.end()) return it->second;
-    return default_grid(A, A->num_cus);
+    return default_grid(A, A->num_cus, mixed_cg);
 }
 
 // Launch a persistent kernel `k` with argument struct `args` on G workgroups of B
