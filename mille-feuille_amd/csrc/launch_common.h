@@ -36,16 +36,16 @@ inline long long watchdog_cycles(double ms)
     return (long long)(ms * 1e-3 * 2.5e9);  // s_memtime ticks at most at the shader clock
 }
 
-int default_grid(const mf_matrix *A, int maxG, bool mixed_cg = false);
+int default_grid(const mf_matrix *A, int maxG, int cg_rows = 0);
 
 // Pick the grid for a persistent kernel of `kind` (autotuned value if present).
-// mixed_cg: the grid of a tile-mixed CG solve (sliced-ELL, row sums in registers).
-inline int choose_grid(mf_matrix *A, const mf_solve_options *o, long long key, bool mixed_cg = false)
+// cg_rows: rows per workgroup of a single-system CG grid (0: the chunk rule).
+inline int choose_grid(mf_matrix *A, const mf_solve_options *o, long long key, int cg_rows = 0)
 {
     if (o->grid > 0) return o->grid;
     auto it = A->tuned_grid.find(key);
     if (it != A->tuned_grid.end()) return it->second;
-    return default_grid(A, A->num_cus, mixed_cg);
+    return default_grid(A, A->num_cus, cg_rows);
 }
 
 // Launch a persistent kernel `k` with argument struct `args` on G workgroups of B

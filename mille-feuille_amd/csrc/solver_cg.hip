@@ -171,13 +171,14 @@ extern "C" void mf_solve_options_default(mf_solve_options *o)
 
 namespace mf {
 
-int default_grid(const mf_matrix *A, int maxG, bool mixed_cg)
+int default_grid(const mf_matrix *A, int maxG, int cg_rows)
 {
-    // tile-mixed CG: about 3072 rows per workgroup (3 to 4 rows per thread), a multiple of
-    // 8 workgroups; its sliced-ELL SpMV keeps pace with fewer, fuller workgroups, and each
-    // grid reduction gets cheaper (n = 262,144: 2.66 ms on 96 workgroups, 3.04 ms on 256)
-    if (mixed_cg) {
-        long long g = ((int64_t)A->L.n_rows + 3071) / 3072;
+    // single-system CG: cg_rows rows per workgroup (3072 tile-mixed, 1536 FP64), a
+    // multiple of 8 workgroups, at least 64; fewer, fuller workgroups make each grid
+    // reduction cheaper while the SpMV keeps pace (n = 262,144, tile-mixed: 2.66 ms on 96
+    // workgroups, 3.04 ms on 256; FP64: 3.09 ms on 192, 3.24 ms on 256)
+    if (cg_rows > 0) {
+        long long g = ((int64_t)A->L.n_rows + cg_rows - 1) / cg_rows;
         g = (g + 7) / 8 * 8;
         g = std::max<long long>(g, 64);
         g = std::max<long long>(1, std::min<long long>(g, maxG));
@@ -205,7 +206,9 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     // else 2); dynamic classes run variant 2
     const int req = dyn ? 2 : (o->variant >= 1 && o->variant <= 4 ? o->variant : (tv >= 1 && tv <= 4 ? tv : 0));
     if (A->L.tilem == 0) return MF_ERR_INVALID;
-    const int G = choose_grid(A, o, ((long long)(req ? req : 2) << 8) | (mixed ? 1 : 0), mixed && !dyn);
+    // (pipelined CG holds five row vectors: at most 2 rows per thread, so the chunk rule)
+    const int G = choose_grid(A, o, ((long long)(req ? req : 2) << 8) | (mixed ? 1 : 0),
+                              dyn || req == 3 ? 0 : (mixed ? 3072 : 1536));
     if (G > A->L.tilem) return MF_ERR_INVALID;
     const Partition *P;
     mf_status st = A->L.partition(G, s, &P);
