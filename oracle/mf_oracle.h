@@ -16,11 +16,17 @@
  *                        16x16 block from the lists and comparing with the CSR input.
  *   oracle_tile_spmv     scipy.sparse CSR matvec (1e-14 relative) and exact integer
  *                        cases.
- *   oracle_cg            numpy.linalg.solve on small SPD systems, scipy.sparse.linalg.cg
- *                        iteration counts (same textbook recurrence), n-step termination
- *                        on tiny SPD systems; a planted sign error must fail those checks.
- *   oracle_bicgstab      numpy.linalg.solve on small nonsymmetric systems and
- *                        scipy.sparse.linalg.bicgstab iteration counts.
+ *   oracle_cg            scipy.sparse.linalg.cg (an independent implementation of the
+ *                        same recurrence): iteration count within one, residual history
+ *                        over the first ten steps to 1e-8, the capped iterate after 7
+ *                        steps; spsolve within kappa * tol; termination in at most k
+ *                        steps on diagonal matrices with k distinct eigenvalues
+ *                        (tests/test_oracle_pins.py).  Planted errors in beta or in the
+ *                        p update fail these checks.
+ *   oracle_bicgstab      scipy.sparse.linalg.bicgstab the same way on convection-diffusion
+ *                        and random nonsymmetric systems, capped iterates after 1, 3 and
+ *                        6 steps, spsolve within cond * tol, and termination in at most
+ *                        k steps on diagonal matrices with k distinct eigenvalues.
  *   oracle_dyn_class     closed-form threshold table at and around every boundary.
  *   oracle_static_class  closed-form representability checks (integers, fp16/fp32
  *                        exactness, subnormals, range limits).

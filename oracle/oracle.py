@@ -13,8 +13,14 @@ NumPy-only pieces:
   tile read at its precision class, the gathered vector converted to binary32,
   products formed in binary32, summed in binary64), restating the per-tile
   ``switch(d_vis[colid])`` of main-cg-mixed.cu:1130-1214 with binary32 products
-  instead of the reference's half/int8 accumulators.  Pinned by exact-integer
-  cases (all products exact) and by its FP64 class reducing to ``tile_spmv``.
+  instead of the reference's half/int8 accumulators.  Pinned in
+  tests/test_oracle_pins.py: bit-equal to the CSR and FP64 tile products on
+  integer data (every product exact), within 2^-24 sum|a x| of SciPy's product
+  of the operands pre-rounded to the tile's precision (and measurably different
+  from the unrounded product), class-0 tiles equal to the FP64 product.
+* ``dyn_classes_rel`` -- the build's magnitude rule for dynamic classes; pinned
+  at and around its thresholds 2^-13 and 2^-24, a ragged last segment and
+  rms = 0 (tests/test_oracle_pins.py).
 * ``static_class_np`` -- vectorised restatement of ``oracle_static_class`` used to
   cross-check the C one on many values.
 """
@@ -328,7 +334,8 @@ def mixed_tile_spmv(d, x, classes, policy="mixed"):
                 if c == 1:
                     v = vals[a:b].astype(np.float32)
                 elif c == 2:
-                    v = vals[a:b].astype(np.float16).astype(np.float32)
+                    # binary16 copy of a matrix value rounds through binary32 (P4)
+                    v = vals[a:b].astype(np.float32).astype(np.float16).astype(np.float32)
                 else:
                     v = vals[a:b].astype(np.float32)
                 prod = (v * x32[gc]).astype(np.float64)
