@@ -37,6 +37,7 @@ class SparseOperator:
         v = np.ascontiguousarray(vals, np.float64)
         self.A = TiledMatrix(self.rowptr, self.colidx, v, class_policy=class_policy, device=device)
         self._vals = v.copy()
+        self.version = 0  # bumped whenever the matrix's values change
         self.AT = None
         if not self.spd:
             # transposed pattern and the permutation from its CSR order to ours
@@ -58,6 +59,7 @@ class SparseOperator:
         if np.array_equal(v, self._vals):
             return
         self._vals = v.copy()
+        self.version += 1
         self.A.update_values(v)
         if self.AT is not None:
             self.AT.update_values(v[self.t_perm])
@@ -89,13 +91,18 @@ def _make_function():
             op.set_values(vals)
             x = op.solve(b)
             ctx.op = op
-            ctx.save_for_backward(x)
+            ctx.version = op.version
+            # vals is kept (not copied) so that backward can restore A(vals) when another
+            # solve changed the shared operator's values in between
+            ctx.save_for_backward(x, vals)
             return x
 
         @staticmethod
         def backward(ctx, g):
-            (x,) = ctx.saved_tensors
+            x, vals = ctx.saved_tensors
             op = ctx.op
+            if op.version != ctx.version:
+                op.set_values(vals)
             lam = op.solve(g.contiguous(), transpose=True)
             grad_vals = op.grad_values(lam, x) if ctx.needs_input_grad[0] else None
             grad_b = lam if ctx.needs_input_grad[1] else None

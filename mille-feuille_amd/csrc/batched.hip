@@ -169,7 +169,9 @@ struct mf_batch {
     // captured launch
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    long long graph_key = -1;
+    // everything the captured graph bakes in (the kernel, its arguments, launch shape and
+    // the reset's extent); a replay needs an identical key, else the graph is re-captured
+    std::vector<unsigned char> graph_key;
     ~mf_batch();
     void free_plan();
 };
@@ -184,7 +186,7 @@ void mf_batch::free_plan()
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
-    graph_key = -1;
+    graph_key.clear();
     planned = false;
 }
 
@@ -419,6 +421,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     if (B->G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
     if ((st = A->ws.ensure(A->L.n_rows, B->G, s)) != MF_SUCCESS) return st;
     mfk::BatchArgs a;
+    memset(&a, 0, sizeof(a));  // the graph key compares the bytes, padding included
     a.A = A->L.dev(B->part);
     a.A.rows_cap = B->rows_per_wg;
     a.sys_rng0 = B->d_sys_rng0;
@@ -447,42 +450,72 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o.max_replacements > 0 ? o.max_replacements : 64;
     const long long nrec = (long long)mfk::kRecWordsPerMember * B->G;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MF_ERR_HIP;
-    const long long key = ((long long)(mixed ? 1 : 0) << 40) ^ (long long)(intptr_t)d_b ^ ((long long)(intptr_t)d_x << 1) ^
-                          ((long long)o.max_iter << 20) ^ (long long)(o.tol * 1e15);
+    struct GraphKey {
+        mfk::BatchArgs a;
+        BatchKernel k;
+        size_t lds;
+        long long nrec;
+        int G, npools;
+    } gk;
+    memset(&gk, 0, sizeof(gk));
+    gk.a = a;
+    gk.k = k;
+    gk.lds = lds;
+    gk.nrec = nrec;
+    gk.G = B->G;
+    gk.npools = B->npools;
+    const unsigned char *kb = (const unsigned char *)&gk;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](mf_status r) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return r;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return done(MF_ERR_HIP);
     (void)hipEventRecord(e0, s);
     if (use_graph) {
-        if (B->graph_key != key) {
+        if (B->graph_key.size() != sizeof(gk) || memcmp(B->graph_key.data(), kb, sizeof(gk)) != 0) {
             if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
             if (B->graph) (void)hipGraphDestroy(B->graph);
             B->gexec = nullptr;
             B->graph = nullptr;
+            B->graph_key.clear();
             hipStream_t cs;
-            if (hipStreamCreate(&cs) != hipSuccess) return MF_ERR_HIP;
-            if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return MF_ERR_HIP;
+            if (hipStreamCreate(&cs) != hipSuccess) return done(MF_ERR_HIP);
+            if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                (void)hipStreamDestroy(cs);
+                return done(MF_ERR_HIP);
+            }
             hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, cs, A->ws.recs, nrec, A->ws.ctl,
                                B->d_queue, B->npools);
             hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, cs, a);
-            if (hipStreamEndCapture(cs, &B->graph) != hipSuccess) return MF_ERR_HIP;
+            const hipError_t ce = hipStreamEndCapture(cs, &B->graph);
             (void)hipStreamDestroy(cs);
-            if (hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0) != hipSuccess) return MF_ERR_HIP;
-            B->graph_key = key;
+            if (ce != hipSuccess) {
+                B->graph = nullptr;
+                return done(MF_ERR_HIP);
+            }
+            if (hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(B->graph);
+                B->graph = nullptr;
+                B->gexec = nullptr;
+                return done(MF_ERR_HIP);
+            }
+            B->graph_key.assign(kb, kb + sizeof(gk));
             (void)hipEventRecord(e0, s);
         }
-        if (hipGraphLaunch(B->gexec, s) != hipSuccess) return MF_ERR_HIP;
+        if (hipGraphLaunch(B->gexec, s) != hipSuccess) return done(MF_ERR_HIP);
     } else {
         hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl,
                            B->d_queue, B->npools);
         hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, s, a);
-        if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+        if (hipGetLastError() != hipSuccess) return done(MF_ERR_HIP);
     }
     (void)hipEventRecord(e1, s);
-    if (hipEventSynchronize(e1) != hipSuccess) return MF_ERR_HIP;
+    if (hipEventSynchronize(e1) != hipSuccess) return done(MF_ERR_HIP);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)done(MF_SUCCESS);
     if (time_ms) *time_ms = ms;
     std::vector<double> h((size_t)8 * B->count);
     if (hipMemcpy(h.data(), B->d_info, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)

@@ -3,8 +3,9 @@
 //   main-cg-mixed   <A.mtx|A.cbd> <max_iter_mix> [out.csv]  (main-cg-mixed.cu:3071-3145)
 //   main-bicg       <A.mtx|A.cbd>                           (main-bicg.cu:1696-1726)
 //   main-bicg-mixed <A.mtx|A.cbd>                           (main-bicg-mixed.cu:2168-2193)
-// As in the reference, rows are truncated to a multiple of 16 and b = A * ones for CG;
-// BiCGSTAB overwrites its right-hand side with the row sums.  CG runs 3 warm-up and 10
+// As in the reference, CG truncates the rows to a multiple of 16 and takes b = A * ones;
+// BiCGSTAB keeps every row, overwrites its right-hand side with the row sums and picks
+// bicgstab_inc / bicgstab_sync / bicgstab by nnz as main-bicg.cu:1696-1726 does.  CG runs 3 warm-up and 10
 // timed solves of exactly max_iter iterations and appends
 // matrix,variant,iterations,time_ms,nnz,l2_norm,residual.
 // Environment: MF_TOL=<tol> runs CG to that relative residual instead of a fixed
@@ -48,10 +49,10 @@ int main(int argc, char **argv)
         return 0;
     }
     printf("matrix rows=%d cols=%d nnz=%d\n", m, n, nnz);
-    const char *nt = getenv("MF_NO_TRUNCATE");
     const int ori = n;
-    if (!(nt && atoi(nt))) n = (n / MF_TILE) * MF_TILE;
 #ifdef MF_CLI_CG
+    const char *nt = getenv("MF_NO_TRUNCATE");
+    if (!(nt && atoi(nt))) n = (n / MF_TILE) * MF_TILE;
     const int max_iter = atoi(argv[2]);
     std::vector<float> vlow(nnz);
     for (int i = 0; i < nnz; i++) vlow[i] = (float)va[i];
@@ -125,8 +126,9 @@ int main(int argc, char **argv)
 #if MF_CLI_MIXED
     bicgstab_mixed(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
 #else
-    if (nnz >= 100000) bicgstab(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
-    else bicgstab_sync(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
+    if (nnz < 10000) bicgstab_inc(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
+    else if (nnz < 100000) bicgstab_sync(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
+    else bicgstab(rp, ci, va, rhs.data(), x.data(), nnz, filename, n);
 #endif
     (void)ori;
 #endif

@@ -49,12 +49,14 @@ inline int choose_grid(mf_matrix *A, const mf_solve_options *o, long long key, i
 }
 
 // Launch a persistent kernel `k` with argument struct `args` on G workgroups of B
-// threads and `lds` bytes of dynamic LDS after checking co-residency; times the
-// launch with HIP events on the caller's stream and copies back the 32-double info
-// block.  Returns MF_ERR_OCCUPANCY instead of launching an oversubscribed grid.
+// threads and `lds` bytes of dynamic LDS after checking co-residency, with the
+// synchronisation words and info block of `w` cleared first by a kernel; times the
+// launch with HIP events on the caller's stream when ms_out is given (the events are
+// created here), then copies back the 32-double info block.  Returns MF_ERR_OCCUPANCY
+// instead of launching an oversubscribed grid.
 template <class Args>
-mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int G, int B, size_t lds,
-                            hipStream_t s, float *ms_out, double *info_out)
+mf_status launch_persistent_ws(mf_matrix *A, const WsView &w, void (*k)(Args), const Args &args, int G, int B,
+                               size_t lds, hipStream_t s, float *ms_out, double *info_out)
 {
     if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return MF_ERR_HIP;
@@ -62,8 +64,16 @@ mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, B, lds) != hipSuccess) return MF_ERR_HIP;
     if (G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
     const long long nrec = mfk::rec_words(G);
-    hipLaunchKernelGGL(mfk::reset_sync_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl, A->ws.info,
-                       A->ws.flags, (long long)mfk::kFlagStride * G);
+    hipLaunchKernelGGL(mfk::reset_sync_kernel, dim3(1), dim3(256), 0, s, w.recs, nrec, w.ctl, w.info, w.flags,
+                       (long long)mfk::kFlagStride * G);
+    if (!ms_out) {
+        hipLaunchKernelGGL(k, dim3(G), dim3(B), lds, s, args);
+        if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+        if (hipMemcpyAsync(info_out, w.info, sizeof(double) * 32, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return MF_ERR_HIP;
+        return MF_SUCCESS;
+    }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return MF_ERR_HIP;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -81,9 +91,17 @@ mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != MF_SUCCESS) return st;
-    if (hipMemcpy(info_out, A->ws.info, sizeof(double) * 32, hipMemcpyDeviceToHost) != hipSuccess) return MF_ERR_HIP;
+    if (hipMemcpy(info_out, w.info, sizeof(double) * 32, hipMemcpyDeviceToHost) != hipSuccess) return MF_ERR_HIP;
     *ms_out = ms;
     return MF_SUCCESS;
+}
+
+// The same with the matrix's own workspace.
+template <class Args>
+mf_status launch_persistent(mf_matrix *A, void (*k)(Args), const Args &args, int G, int B, size_t lds,
+                            hipStream_t s, float *ms_out, double *info_out)
+{
+    return launch_persistent_ws(A, ws_view(A->ws), k, args, G, B, lds, s, ms_out, info_out);
 }
 
 }  // namespace mf

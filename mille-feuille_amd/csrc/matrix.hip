@@ -681,10 +681,12 @@ void sell_bytes(const Partition *P, double *class_streams, double *all64)
 }
 
 mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
-                      bool *narrow, bool allow_perm)
+                      bool *narrow, bool allow_perm, bool build)
 {
     // the sliced-ELL streams are a cache of the partition, rebuilt with it
     Partition &P = const_cast<Partition &>(*Pc);
+    if (!build && (!P.sell || P.sell->rpt != rpt || !P.sell->soff[mfk::kSellAll64] || (mixed && !P.sell->mixed)))
+        return MF_ERR_UNSUPPORTED;
     if (P.sell && P.sell->rpt != rpt) {
         free_sell(P.sell);
         P.sell = nullptr;

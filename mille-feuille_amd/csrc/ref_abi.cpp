@@ -98,7 +98,7 @@ void cg_common(int *RowPtr, int *ColIdx, double *Val, double *x, double *b, int 
 }
 
 int bicg_common(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR, char *filename, int n,
-                int iterations, int precision)
+                int iterations, int precision, Path path)
 {
     if (n <= 0) return 0;
     std::vector<int> rp, ci;
@@ -121,12 +121,28 @@ int bicg_common(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, i
     o.precision = precision;
     mf_solve_info info;
     memset(&info, 0, sizeof(info));
-    mf_bicgstab_solve_host(A, rhs, x, &o, &info);
+    mf_status st = MF_ERR_TOO_LARGE;
+    if (path == PERSISTENT) st = mf_bicgstab_solve_host(A, rhs, x, &o, &info);
+    if (path == MULTI || st == MF_ERR_TOO_LARGE || st == MF_ERR_OCCUPANCY) {
+        // host loop with one launch per step (main-bicg.cu:1336's bicgstab)
+        double *db = nullptr, *dx = nullptr;
+        if (hipMalloc((void **)&db, sizeof(double) * n) == hipSuccess &&
+            hipMalloc((void **)&dx, sizeof(double) * n) == hipSuccess &&
+            hipMemcpy(db, rhs, sizeof(double) * n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemset(dx, 0, sizeof(double) * n) == hipSuccess) {
+            mf_bicgstab_solve_multikernel(A, db, dx, &o, &info, nullptr);
+            (void)hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost);
+        }
+        if (db) (void)hipFree(db);
+        if (dx) (void)hipFree(dx);
+    }
     double bn = 0;
     for (int i = 0; i < n; i++) bn += rhs[i] * rhs[i];
     bn = std::sqrt(bn);
     printf("time_bicg=%lf ms\n", info.time_ms);
-    append_csv("bicg_performance.csv", filename, info.iterations, info.time_ms, nnzR, info.relres_true,
+    // the reference appends to data/bicg_performance.csv (main-bicg.cu:678-688) and prints
+    // "open error!" when the folder is missing; the iteration column holds the count run
+    append_csv("data/bicg_performance.csv", filename, info.iterations, info.time_ms, nnzR, info.relres_true,
                info.relres * bn);
     mf_matrix_destroy(A);
     return 0;
@@ -182,23 +198,23 @@ extern "C" void cg_solve_inc_mixed(CG_ARGS)
 
 extern "C" int bicgstab(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR, char *filename, int n)
 {
-    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 10000, MF_PREC_FP64);
+    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 10000, MF_PREC_FP64, MULTI);
 }
 
 extern "C" int bicgstab_sync(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR, char *filename,
                              int n)
 {
-    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_FP64);
+    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_FP64, PERSISTENT);
 }
 
 extern "C" int bicgstab_inc(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR, char *filename,
                             int n)
 {
-    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_FP64);
+    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_FP64, PERSISTENT);
 }
 
 extern "C" int bicgstab_mixed(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR,
                               char *filename, int n)
 {
-    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_MIXED);
+    return bicg_common(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, 50, MF_PREC_MIXED, PERSISTENT);
 }

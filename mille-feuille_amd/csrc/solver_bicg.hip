@@ -538,7 +538,11 @@ extern "C" mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *
     mf_solve_options_default(&o);
     if (opt) o = *opt;
     if (o.transpose) return MF_ERR_UNSUPPORTED;
-    return bicg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    const mf_status st = bicg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    // beyond the persistent grid's capacity: the multi-kernel recurrence (info->variant = 0)
+    if ((st == MF_ERR_TOO_LARGE || st == MF_ERR_OCCUPANCY) && o.grid == 0)
+        return bicg_multikernel(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    return st;
 }
 
 extern "C" mf_status mf_bicgstab_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,

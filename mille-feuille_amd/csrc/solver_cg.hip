@@ -176,8 +176,10 @@ int default_grid(const mf_matrix *A, int maxG, int cg_rows)
     // single-system CG: cg_rows rows per workgroup (3072 tile-mixed, 1536 FP64), a
     // multiple of 8 workgroups, at least 64; fewer, fuller workgroups make each grid
     // reduction cheaper while the SpMV keeps pace (n = 262,144, tile-mixed: 2.66 ms on 96
-    // workgroups, 3.04 ms on 256; FP64: 3.09 ms on 192, 3.24 ms on 256)
-    if (cg_rows > 0) {
+    // workgroups, 3.04 ms on 256; FP64: 3.09 ms on 192, 3.24 ms on 256).  The rule was
+    // measured on stencils; matrices with more than 16 entries per row keep the chunk rule
+    // below, whose bandwidth-bound SpMV wants every CU.
+    if (cg_rows > 0 && A->L.nnz <= 16LL * A->L.n_rows) {
         long long g = ((int64_t)A->L.n_rows + cg_rows - 1) / cg_rows;
         g = (g + 7) / 8 * 8;
         g = std::max<long long>(g, 64);
@@ -196,8 +198,11 @@ int default_grid(const mf_matrix *A, int maxG, int cg_rows)
     return (int)g;
 }
 
+// ext: run on these buffers instead of the matrix's workspace, building nothing (the
+// partition and SpMV streams must exist from an earlier solve) and allocating nothing
+// (no workspace growth, no timing events: info->time_ms is 0).
 mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *o, mf_solve_info *info,
-                    hipStream_t s)
+                    hipStream_t s, const WsView *ext = nullptr)
 {
     const bool dyn = o->precision == MF_PREC_MIXED && o->dynamic_precision != 0 && A->L.has_copies;
     const bool mixed = o->precision == MF_PREC_MIXED && (A->L.has_lowp || dyn);
@@ -210,9 +215,13 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     const int G = choose_grid(A, o, ((long long)(req ? req : 2) << 8) | (mixed ? 1 : 0),
                               dyn || req == 3 ? 0 : (mixed ? 3072 : 1536));
     if (G > A->L.tilem) return MF_ERR_INVALID;
-    const Partition *P;
-    mf_status st = A->L.partition(G, s, &P);
-    if (st != MF_SUCCESS) return st;
+    const Partition *P = nullptr;
+    mf_status st = MF_SUCCESS;
+    if (ext) {
+        if (!(P = A->L.find_partition(G))) return MF_ERR_UNSUPPORTED;
+    } else if ((st = A->L.partition(G, s, &P)) != MF_SUCCESS) {
+        return st;
+    }
     const int var = req ? req : (P->max_deps <= kAutoMaxDeps ? 4 : 2);
     int rpt = 0, B = 0;
     if (!shape_for(P->rows_cap, rpt, B)) return MF_ERR_TOO_LARGE;
@@ -234,11 +243,15 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     }
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
     if (var == 3 && rpt > 2) return MF_ERR_UNSUPPORTED;
-    if ((st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
+    if (ext && ext->cap_g > 0 && G > ext->cap_g) return MF_ERR_TOO_LARGE;
+    if (!ext && (st = A->ws.ensure(std::max(A->L.n_rows, A->L.n_cols), G, s)) != MF_SUCCESS) return st;
+    const WsView w = ext ? *ext : ws_view(A->ws);
+    if (ext && var == 3) return MF_ERR_UNSUPPORTED;  // no second publish buffer pair in the scratch
     mfk::DevSell dsell;
     memset(&dsell, 0, sizeof(dsell));
     bool narrow = false;
-    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow, cg_perm_supported(rpt, var))) != MF_SUCCESS)
+    if (sell && (st = ensure_sell(A, P, rpt, mixed, s, &dsell, &narrow, cg_perm_supported(rpt, var), !ext)) !=
+                    MF_SUCCESS)
         return st;
     // Tile-mixed solve whose class streams would move about as many bytes as the binary64
     // stream (rows spread over classes pad each class stream): binary64 arithmetic on the
@@ -265,17 +278,17 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
 
     mfk::CgArgs a;
     a.A = A->L.dev(*P);
-    a.gs.recs = A->ws.recs;
-    a.gs.ctl = A->ws.ctl;
+    a.gs.recs = w.recs;
+    a.gs.ctl = w.ctl;
     a.gs.timeout = watchdog_cycles(o->watchdog_ms);
     a.gs.G = G;
     a.b = d_b;
     a.x = d_x;
-    a.p64 = A->ws.p64;
-    a.p32 = A->ws.p32;
-    a.p64b = A->ws.s64;
-    a.p32b = A->ws.s32;
-    a.info = A->ws.info;
+    a.p64 = w.p64;
+    a.p32 = w.p32;
+    a.p64b = w.s64;
+    a.p32b = w.s32;
+    a.info = w.info;
     a.tol2 = o->tol * o->tol;
     const double dl = o->replace_delta > 0 ? o->replace_delta : 1e-3;
     a.delta2 = dl * dl;
@@ -285,17 +298,17 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.need64 = (!mixed_arith || A->L.need_p64_mixed) ? 1 : 0;
     a.need32 = mixed_arith ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
-    a.dvis = A->ws.dvis;
+    a.dvis = w.dvis;
     a.dyn_mode = dyn ? (o->dynamic_precision == 2 ? 2 : 1) : 0;
-    a.A.dvis = dyn ? A->ws.dvis : nullptr;
+    a.A.dvis = dyn ? w.dvis : nullptr;
     if (dyn) a.need64 = 1;  // dynamic classes may keep any tile at binary64
     a.S = dsell;
-    a.nb.flags = A->ws.flags;
+    a.nb.flags = w.flags;
     a.nb.dep_ptr = P->dep_ptr;
     a.nb.dep_idx = P->dep_idx;
     float ms = 0;
     double h[32];
-    if ((st = launch_persistent(A, k, a, G, B, lds, s, &ms, h)) != MF_SUCCESS) return st;
+    if ((st = launch_persistent_ws(A, w, k, a, G, B, lds, s, ext ? nullptr : &ms, h)) != MF_SUCCESS) return st;
     const double bb = h[2];
     if (info) {
         memset(info, 0, sizeof(*info));
@@ -330,7 +343,13 @@ extern "C" mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, c
     mf_solve_options_default(&o);
     if (opt) o = *opt;
     if (o.transpose) return MF_ERR_UNSUPPORTED;
-    return cg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    const mf_status st = cg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    // a system the persistent grid cannot hold (more rows than co-resident workgroups
+    // keep in registers, e.g. beyond 2,097,152 rows on 256 CUs) runs the multi-kernel
+    // recurrence instead (info->variant = 0)
+    if ((st == MF_ERR_TOO_LARGE || st == MF_ERR_OCCUPANCY) && o.grid == 0)
+        return cg_multikernel(A, d_b, d_x, &o, info, (hipStream_t)stream);
+    return st;
 }
 
 extern "C" mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, const mf_solve_options *opt,
@@ -398,3 +417,4 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
     if (best_ms_per_iter) *best_ms_per_iter = best;
     return MF_SUCCESS;
 }
+

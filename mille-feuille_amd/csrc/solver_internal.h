@@ -79,6 +79,12 @@ struct DeviceLayout {
     ~DeviceLayout();
     void release();
     mf_status partition(int G, hipStream_t s, const Partition **out);
+    // The partition over G ranges if an earlier call built it, else null (builds nothing).
+    const Partition *find_partition(int G) const
+    {
+        auto it = parts.find(G);
+        return it == parts.end() ? nullptr : &it->second;
+    }
     // Build a Partition over the given tile-row cut points (cuts[0] = 0, cuts.back() =
     // tilem, non-decreasing).  The caller owns the result (free_partition).
     mf_status build_ranges(const std::vector<int> &cuts, hipStream_t s, Partition &P) const;
@@ -93,8 +99,10 @@ bool sell_supported(const mf_matrix *A, int rpt);
 // use; `mixed` also builds the per-class streams); *narrow receives their unit profile and
 // out->perm the row permutation (null: identity), which the first build may choose only
 // when allow_perm (the caller has kernels for it).
+// With build = false nothing is built or freed: MF_ERR_UNSUPPORTED when the streams an
+// earlier call built do not cover (rpt, mixed).
 mf_status ensure_sell(mf_matrix *A, const Partition *P, int rpt, bool mixed, hipStream_t s, mfk::DevSell *out,
-                      bool *narrow, bool allow_perm);
+                      bool *narrow, bool allow_perm, bool build = true);
 // Padded bytes of one pass over the built class streams (0..3) and over the binary64
 // stream (4) of P's sliced-ELL layout.
 void sell_bytes(const Partition *P, double *class_streams, double *all64);
@@ -118,6 +126,42 @@ struct Workspace {
     mf_status ensure(int64_t n, int G, hipStream_t s);
     int64_t cap_dvis = 0;
 };
+
+// Device buffers one persistent solve launch uses besides the matrix and the caller's
+// vectors: the publish buffers, the grid-reduction records, publish flags, the control
+// word, the 32-double info block and the dynamic classes.  Taken from a matrix's
+// Workspace, or from the library's static scratch (mf_cg_solve_noalloc).
+struct WsView {
+    double *p64 = nullptr, *s64 = nullptr;
+    float *p32 = nullptr, *s32 = nullptr;
+    unsigned long long *recs = nullptr;
+    unsigned *flags = nullptr;
+    unsigned *ctl = nullptr;
+    uint8_t *dvis = nullptr;
+    double *info = nullptr;
+    int cap_g = 0;  // most workgroups the records and flags hold (0: grown on demand)
+};
+inline WsView ws_view(const Workspace &w)
+{
+    WsView v;
+    v.p64 = w.p64;
+    v.s64 = w.s64;
+    v.p32 = w.p32;
+    v.s32 = w.s32;
+    v.recs = w.recs;
+    v.flags = w.flags;
+    v.ctl = w.ctl;
+    v.dvis = w.dvis;
+    v.info = w.info;
+    return v;
+}
+
+// Multi-kernel CG / BiCGSTAB (solver_multi.hip): no co-residency requirement; also the
+// path of mf_cg_solve / mf_bicgstab_solve for systems the persistent grid cannot hold.
+mf_status cg_multikernel(mf_matrix *A, const double *b, double *x, const mf_solve_options *o, mf_solve_info *info,
+                         hipStream_t s);
+mf_status bicg_multikernel(mf_matrix *A, const double *b, double *x, const mf_solve_options *o,
+                           mf_solve_info *info, hipStream_t s);
 
 mf_status upload(void **dst, const void *src, size_t bytes, hipStream_t s);
 void free_dev(void *p);

@@ -60,19 +60,34 @@ __global__ __launch_bounds__(256) void mk_finish(const double *__restrict__ part
     if (threadIdx.x == 0) *out = s[0] + s[1] + s[2] + s[3];
 }
 
-// y = a*x + y with a = sc[num]/sc[den] (sign applied), optional r'r partial of y
+// Whether every scalar slot named in `mask` (bit k: sc[k]) is finite and nonzero.  The
+// vector updates of an iteration do nothing when a denominator broke down, so the
+// iterate the host finds on MF_BREAKDOWN is the last finite one.
+__device__ __forceinline__ bool mk_ok(const double *sc, unsigned mask)
+{
+    for (int k = 0; k < 8; k++)
+        if ((mask >> k) & 1u) {
+            const double d = sc[k];
+            if (!(d != 0.0) || !isfinite(d)) return false;
+        }
+    return true;
+}
+
+// y = a*x + y with a = sc[num]/sc[den] (sign applied); nothing when sc[den] broke down
 __global__ __launch_bounds__(256) void mk_axpy(const double *__restrict__ sc, int num, int den, double sign,
                                                const double *__restrict__ x, double *__restrict__ y, int n)
 {
+    if (!mk_ok(sc, 1u << den)) return;
     const double a = sign * sc[num] / sc[den];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] += a * x[i];
 }
 
-// p = r + (sc[num]/sc[den]) p, plus binary32 copy
-__global__ __launch_bounds__(256) void mk_xpay(const double *__restrict__ sc, int num, int den,
+// p = r + (sc[num]/sc[den]) p, plus binary32 copy; nothing when a slot of `guard` broke down
+__global__ __launch_bounds__(256) void mk_xpay(const double *__restrict__ sc, int num, int den, unsigned guard,
                                                const double *__restrict__ r, double *__restrict__ p,
                                                float *__restrict__ p32, int n)
 {
+    if (!mk_ok(sc, guard)) return;
     const double beta = sc[num] / sc[den];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double v = r[i] + beta * p[i];
@@ -108,6 +123,7 @@ __global__ void mk_residual(const double *b, const double *q, double *r, double 
 __global__ void mk_bicg_s(const double *__restrict__ sc, const double *__restrict__ r, const double *__restrict__ v,
                           double *__restrict__ s, float *__restrict__ s32, int n)
 {
+    if (!mk_ok(sc, 1u << 1)) return;
     const double alpha = sc[0] / sc[1];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double sv = r[i] - alpha * v[i];
@@ -120,6 +136,7 @@ __global__ void mk_bicg_xr(const double *__restrict__ sc, double *__restrict__ x
                            const double *__restrict__ p, const double *__restrict__ s, const double *__restrict__ t,
                            int n)
 {
+    if (!mk_ok(sc, (1u << 1) | (1u << 3))) return;
     const double alpha = sc[0] / sc[1], omega = sc[2] / sc[3];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i] + omega * s[i];
@@ -130,6 +147,7 @@ __global__ void mk_bicg_xr(const double *__restrict__ sc, double *__restrict__ x
 __global__ void mk_bicg_p(double *__restrict__ sc, double *__restrict__ p, float *__restrict__ p32,
                           const double *__restrict__ r, const double *__restrict__ v, int n)
 {
+    if (!mk_ok(sc, (1u << 0) | (1u << 1) | (1u << 2) | (1u << 3))) return;
     const double alpha = sc[0] / sc[1], omega = sc[2] / sc[3];
     const double beta = (sc[4] / sc[0]) * (alpha / omega);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -296,7 +314,7 @@ mf_status cg_multikernel(mf_matrix *A, const double *b, double *x, const mf_solv
         hipLaunchKernelGGL(mfk::mk_axpy, vg, dim3(256), 0, s, c.sc, 0, 1, 1.0, p, x, n);
         hipLaunchKernelGGL(mfk::mk_axpy, vg, dim3(256), 0, s, c.sc, 0, 1, -1.0, q, r, n);
         c.dot(r, r, 2);
-        hipLaunchKernelGGL(mfk::mk_xpay, vg, dim3(256), 0, s, c.sc, 2, 0, r, p, p32, n);
+        hipLaunchKernelGGL(mfk::mk_xpay, vg, dim3(256), 0, s, c.sc, 2, 0, 1u << 1, r, p, p32, n);
         hipLaunchKernelGGL(mfk::mk_set, dim3(1), dim3(1), 0, s, c.sc, 0, 2);
         const double pq = c.read(1);
         if (!(pq != 0.0) || !std::isfinite(pq)) {

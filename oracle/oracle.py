@@ -343,6 +343,50 @@ def mixed_tile_spmv(d, x, classes, policy="mixed"):
     return y[:rowA]
 
 
+def entry_tile_classes(rowptr, colidx, val, n_cols=None):
+    """Static lossless class of every CSR entry's tile (the minimum over the tile's values,
+    as ``tile_classes``), vectorised for matrices with millions of tiles."""
+    rp = np.asarray(rowptr, np.int64)
+    ci = np.asarray(colidx, np.int64)
+    n = rp.size - 1
+    ncols = int(n_cols) if n_cols is not None else (int(ci.max()) + 1 if ci.size else 1)
+    tilen = (ncols + 15) // 16
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    key = (rows // 16) * tilen + ci // 16
+    per = static_class_np(val)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    start = np.flatnonzero(np.concatenate(([True], ks[1:] != ks[:-1])))
+    mins = np.minimum.reduceat(per[order], start)
+    counts = np.diff(np.append(start, ks.size))
+    out = np.empty_like(per)
+    out[order] = np.repeat(mins, counts)
+    return out
+
+
+def mixed_csr_spmv(rowptr, colidx, val, x, entry_cls):
+    """The tile-mixed product of ``mixed_tile_spmv`` evaluated per CSR entry (vectorised):
+    class 0 entries in binary64, classes 1-3 as binary32 products of the value at its class
+    precision and x rounded to binary32, summed in binary64 (in CSR order per row, so the
+    sums may differ from the tile order's in the last bits)."""
+    rp = np.asarray(rowptr, np.int64)
+    ci = np.asarray(colidx, np.int64)
+    v = np.asarray(val, np.float64)
+    c = np.asarray(entry_cls)
+    n = rp.size - 1
+    xx = np.asarray(x, np.float64)
+    x32 = xx.astype(np.float32)
+    prod = v * xx[ci]
+    low = c != 0
+    vl = v[low]
+    vl32 = vl.astype(np.float32)
+    is16 = c[low] == 2
+    vl32[is16] = vl32[is16].astype(np.float16).astype(np.float32)
+    prod[low] = (vl32 * x32[ci[low]]).astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    return np.bincount(rows, weights=prod, minlength=n)
+
+
 def entry_rows(d):
     """Local row (0..15) of every stored entry, from the per-row start bytes."""
     rowA = d["_rowA"]

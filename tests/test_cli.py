@@ -120,12 +120,27 @@ def test_cg_cli_tolerance_mode(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["main-bicg", "main-bicg-mixed"])
-def test_bicg_cli_runs(tmp_path, name):
-    mtx = str(tmp_path / "poisson32.mtx")
-    _poisson_mtx(mtx, 32)
+@pytest.mark.parametrize("m", [32, 33, 120])
+def test_bicg_cli_runs(tmp_path, name, m):
+    """main-bicg keeps every row (33^2 = 1089 is not a multiple of 16), runs 50 iterations
+    below 10^5 entries and the multi-kernel bicgstab above (120^2 Poisson: 71,520 entries
+    -> bicgstab_sync; the mixed tool always runs 50), and appends
+    path,iterations,time,nnz,l2,residual to data/bicg_performance.csv
+    (main-bicg.cu:678-688, 1696-1726)."""
+    mtx = str(tmp_path / f"poisson{m}.mtx")
+    _poisson_mtx(mtx, m)
+    # without data/ the row cannot be written: the reference prints "open error!"
+    r0 = subprocess.run([_cli(name), mtx], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r0.returncode == 0, r0.stderr
+    assert "open error!" in r0.stdout
+    os.makedirs(tmp_path / "data")
     r = subprocess.run([_cli(name), mtx], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr
-    assert os.path.exists(tmp_path / "bicg_performance.csv") or "bicg" in r.stdout.lower()
+    rows = (tmp_path / "data" / "bicg_performance.csv").read_text().strip().splitlines()
+    assert len(rows) == 1
+    f = rows[0].split(",")
+    assert f[0] == mtx and int(f[1]) == 50 and int(f[3]) == 5 * m * m - 4 * m
+    assert np.isfinite(float(f[4])) and float(f[2]) > 0
 
 
 def _poisson_csr(m):

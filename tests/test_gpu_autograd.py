@@ -95,3 +95,31 @@ def test_adjoint_64cubed_mixed_vs_fp64(mf, torch):
     assert np.linalg.norm(gb1 - gb2) <= 1e-6 * np.linalg.norm(gb2)
     op1.close()
     op2.close()
+
+
+def test_two_forwards_then_backward_use_their_own_values(mf, torch):
+    """Two solves with different values on one shared operator before either backward:
+    each gradient must be that of its own A(vals) (dense FP64 reference)."""
+    import scipy.sparse as sp
+    from mille_feuille_amd.autograd import SparseOperator, sparse_solve
+    rp, ci, va, b, xt = _problem(mf, 8)
+    n = rp.size - 1
+    op = SparseOperator(rp, ci, va, spd=True, precision="fp64")
+    va2 = va * 1.5
+    v1 = torch.tensor(va, device="cuda", requires_grad=True)
+    v2 = torch.tensor(va2, device="cuda", requires_grad=True)
+    bt = torch.as_tensor(b, device="cuda")
+    xtt = torch.as_tensor(xt, device="cuda")
+    x1 = sparse_solve(v1, bt, op)
+    x2 = sparse_solve(v2, bt, op)   # overwrites the operator's values
+    (0.5 * torch.sum((x1 - xtt) ** 2)).backward()
+    (0.5 * torch.sum((x2 - xtt) ** 2)).backward()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    for vals, v in ((va, v1), (va2, v2)):
+        A = sp.csr_matrix((vals, ci, rp), shape=(n, n)).toarray()
+        xr = np.linalg.solve(A, b)
+        lam = np.linalg.solve(A.T, xr - xt)
+        ref = -lam[rows] * xr[ci]
+        got = v.grad.cpu().numpy()
+        assert np.linalg.norm(got - ref) <= 1e-6 * np.linalg.norm(ref)
+    op.close()

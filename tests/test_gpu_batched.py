@@ -87,3 +87,52 @@ def test_value_update_and_pattern_outer(mf, orc, torch):
     rows = np.repeat(np.arange(n), np.diff(rp))
     assert np.array_equal(out, -u.cpu().numpy()[rows] * v.cpu().numpy()[ci])
     A.close()
+
+
+def test_graph_replay_follows_changed_options(mf, orc, torch):
+    """A cached graph must not be replayed after an option it bakes in changes:
+    fixed_iterations, max_iter, max_replacements and the tolerance each re-capture."""
+    systems = _systems(mf, 6, 9)
+    B = mf.Batch([(r, c, v) for r, c, v, _ in systems], class_policy="lossless")
+    b = B.pack([s[3] for s in systems])
+    _, inf_a, _ = B.cg(b, use_graph=True, tol=1e-10, precision="mixed")
+    assert all(i["status"] == 0 for i in inf_a)
+    # fixed budget of 7 iterations: every system stops at exactly 7
+    _, inf_b, _ = B.cg(b, use_graph=True, tol=1e-10, precision="mixed", fixed_iterations=True, max_iter=7)
+    assert [i["iterations"] for i in inf_b] == [7] * len(systems)
+    # back to tolerance mode: the first graph's answer again
+    _, inf_c, _ = B.cg(b, use_graph=True, tol=1e-10, precision="mixed")
+    assert [i["iterations"] for i in inf_c] == [i["iterations"] for i in inf_a]
+    # no residual replacements allowed: the count reported is 0
+    _, inf_d, _ = B.cg(b, use_graph=True, tol=1e-10, precision="mixed", max_replacements=1, replace_delta=1e-30)
+    assert all(i["replacements"] <= 1 for i in inf_d)
+    # a looser tolerance needs fewer iterations
+    _, inf_e, _ = B.cg(b, use_graph=True, tol=1e-4, precision="mixed")
+    assert all(e["iterations"] < a["iterations"] for e, a in zip(inf_e, inf_a))
+    B.close()
+
+
+@pytest.mark.parametrize("fn", ["mf_cg_solve_multikernel", "mf_cg_solve"])
+def test_cg_breakdown_keeps_last_finite_iterate(mf, torch, fn):
+    """p'Ap = 0 at the first step (blocks [[0, 1], [1, 0]], b = e_1): MF_BREAKDOWN, and
+    x keeps its last finite iterate (x0 = 0) instead of Inf/NaN."""
+    import ctypes
+    n = 64
+    rp = np.arange(0, n + 1, dtype=np.int32)
+    ci = np.array([i + 1 if i % 2 == 0 else i - 1 for i in range(n)], np.int32)
+    va = np.ones(n)
+    b = np.zeros(n)
+    b[0::2] = 1.0
+    A = mf.TiledMatrix(rp, ci, va)
+    lb = mf._lib.lib()
+    o = mf.api.solve_options(tol=1e-10, precision="fp64")
+    bt = torch.as_tensor(b, device="cuda:0")
+    x = torch.zeros_like(bt)
+    info = mf._lib.SolveInfo()
+    st = getattr(lb, fn)(A.h, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(x.data_ptr()), ctypes.byref(o),
+                         ctypes.byref(info), None)
+    torch.cuda.synchronize()
+    assert mf._lib.STATUS_NAMES[st] == "MF_BREAKDOWN", (st, info.as_dict())
+    xh = x.cpu().numpy()
+    assert np.all(np.isfinite(xh)) and not xh.any()
+    A.close()

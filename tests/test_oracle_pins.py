@@ -313,3 +313,31 @@ def test_dyn_classes_rel_thresholds(orc):
     # a ragged last segment and rms = 0
     assert list(orc.dyn_classes_rel(np.full(20, 1.0), 1.0)) == [0, 0]
     assert list(orc.dyn_classes_rel(np.full(20, 1.0), 0.0)) == [4, 4]
+
+
+def test_vectorised_mixed_emulation_matches_tile_emulation(orc):
+    """The per-entry vectorised emulation (used at n = 10^6) against the tile-order one:
+    the same classes, products within binary64 summation-order rounding."""
+    rng = np.random.default_rng(31)
+    n = 500
+    A = sp.random(n, n, density=0.03, random_state=rng, format="csr")
+    vals = rng.uniform(-3.0, 3.0, A.data.size)
+    pick = rng.random(A.data.size)
+    vals[pick < 0.3] = -1.0                        # int8-exact
+    vals[(pick >= 0.3) & (pick < 0.5)] = 0.375     # binary16-exact
+    A.data = vals
+    A.sort_indices()
+    rp, ci, va = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    d = _tiles(orc, rp, ci, va)
+    tcls = orc.tile_classes(d)
+    ecls = orc.entry_tile_classes(rp, ci, va)
+    assert set(np.unique(ecls)) >= {0, 3}
+    x = rng.uniform(-1, 1, n)
+    y_t = orc.mixed_tile_spmv(d, x, tcls)
+    y_v = orc.mixed_csr_spmv(rp, ci, va, x, ecls)
+    scale = abs(A) @ np.abs(x)
+    assert np.all(np.abs(y_t - y_v) <= 1e-14 * scale)
+    # the classes follow the tiles: every entry of a tile carries the tile's class
+    tile_of = np.repeat(np.arange(n), np.diff(rp)) // 16 * ((n + 15) // 16) + ci // 16
+    for t in np.unique(tile_of)[:200]:
+        assert np.unique(ecls[tile_of == t]).size == 1
