@@ -446,7 +446,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     a.delta2 = dl * dl;
     a.maxit = o.max_iter;
     a.fixed = o.fixed_iterations;
-    a.need64 = (!mixed || A->L.need_p64_mixed) ? 1 : 0;
+    a.need64 = mixed ? 0 : 1;  // tile-mixed products take the binary32 p (kPmLow32)
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o.max_replacements > 0 ? o.max_replacements : 64;
     const long long nrec = (long long)mfk::kRecWordsPerMember * B->G;

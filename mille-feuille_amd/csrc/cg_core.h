@@ -96,17 +96,21 @@ struct XVec {
     }
 };
 
+// Product mode of the CG kernels' tile-mixed SpMV (sell.h): every class multiplies the
+// binary32 p, so a row's product is that of one consistently rounded vector.
+template <bool MIXED> constexpr int cg_pm() { return MIXED ? kPmLow32 : kPmTiles; }
+
 // q <- A v (tile classes when MIXED) for the vector v published in x64/x32.
 template <bool MIXED, int U, bool DYN, int SELL, int RPT, int BLK>
 __device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, const double *x64, const float *x32,
                                         QVec<RPT, (SELL != 0)> &qv, double *scratch)
 {
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4)>(a.S, a.slot, R.base(), x64, x32, qv.reg);
+        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4), cg_pm<MIXED>()>(a.S, a.slot, R.base(), x64, x32, qv.reg);
     } else {
         zero_q<RPT, BLK>(qv.lds, R);
         __syncthreads();
-        spmv_chunks<MIXED, U, DYN>(a.A, a.cr, R.row0, x64, x32, qv.lds, scratch);
+        spmv_chunks<MIXED, U, DYN, cg_pm<MIXED>()>(a.A, a.cr, R.row0, x64, x32, qv.lds, scratch);
         __syncthreads();
     }
 }
@@ -495,8 +499,9 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     // sliced-ELL: the first units of the next SpMV are loaded while the grid reduction,
     // the update and the flag wait run (the matrix does not change)
     constexpr bool NAR = SELL == 2 || SELL == 4;
-    SellLead<sell_lead_stream<MIXED>(), NAR> lead;
-    if constexpr (SELL != 0) sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
+    constexpr int PM = cg_pm<MIXED>();
+    SellLead<sell_lead_stream<MIXED>(), NAR, PM> lead;
+    if constexpr (SELL != 0) sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
     while (alive && res.status == ST_NOTCONV) {
         if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
             res.status = ST_TIMEOUT;
@@ -505,8 +510,8 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         p_visible = false;
         // q = A p
         if constexpr (SELL != 0) {
-            spmv_sell_lead<MIXED, RPT, BLK, NAR>(a.S, a.slot, R.base(), a.p64, a.p32, q.reg, lead);
-            sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
+            spmv_sell_lead<MIXED, RPT, BLK, NAR, PM>(a.S, a.slot, R.base(), a.p64, a.p32, q.reg, lead);
+            sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
         } else {
             spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
         }

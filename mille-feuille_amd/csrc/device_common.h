@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "sell.h"
+
 namespace mfk {
 
 constexpr int kTile = 16;
@@ -380,10 +382,11 @@ __device__ __forceinline__ auto quad_value(const Q &v, int j)
 // Pipeline registers of one batch of the SpMV: chunk headers, entry indices (4 per
 // lane), values (4 per lane), the gathered x of those 4 entries, and whether each chunk
 // exists (and, under dynamic classes, its column tile's dynamic class).
-template <int CLS, int U>
+template <int CLS, int U, int PM = kPmTiles>
 struct SpmvStage {
     using QT = decltype(load_quad<CLS>(DevMatrix{}, 0));
-    using XT = std::conditional_t<CLS == 0, double, float>;
+    static constexpr bool X32 = pm_x32<PM>(CLS);  // gathers the binary32 copy of x
+    using XT = std::conditional_t<X32, float, double>;
     int3 h[U];
     unsigned id[U];
     QT val[U];
@@ -392,8 +395,8 @@ struct SpmvStage {
     bool v[U];
 };
 
-template <int CLS, int U>
-__device__ __forceinline__ void spmv_load_headers(const DevMatrix &A, SpmvStage<CLS, U> &S, int cb, int last)
+template <int CLS, int U, int PM>
+__device__ __forceinline__ void spmv_load_headers(const DevMatrix &A, SpmvStage<CLS, U, PM> &S, int cb, int last)
 {
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -403,8 +406,8 @@ __device__ __forceinline__ void spmv_load_headers(const DevMatrix &A, SpmvStage<
     }
 }
 
-template <int CLS, int U>
-__device__ __forceinline__ void spmv_load_entries(const DevMatrix &A, SpmvStage<CLS, U> &S, int sub)
+template <int CLS, int U, int PM>
+__device__ __forceinline__ void spmv_load_entries(const DevMatrix &A, SpmvStage<CLS, U, PM> &S, int sub)
 {
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -418,8 +421,8 @@ __device__ __forceinline__ void spmv_load_entries(const DevMatrix &A, SpmvStage<
 // column: in range (the tile exists), inside the line the active lanes read anyway, and
 // never a single address shared by every padded entry of the grid (which concentrated
 // the requests of all CUs on one L2 channel).
-template <int CLS, int U, bool DYN>
-__device__ __forceinline__ void spmv_load_gathers(const DevMatrix &A, SpmvStage<CLS, U> &S, int sub,
+template <int CLS, int U, bool DYN, int PM>
+__device__ __forceinline__ void spmv_load_gathers(const DevMatrix &A, SpmvStage<CLS, U, PM> &S, int sub,
                                                   const double *__restrict__ x64, const float *__restrict__ x32)
 {
 #pragma unroll
@@ -430,8 +433,8 @@ __device__ __forceinline__ void spmv_load_gathers(const DevMatrix &A, SpmvStage<
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int col = cbase + (4 * sub + j <= lastk ? (int)((S.id[u] >> (8 * j)) & 15) : 0);
-            if constexpr (CLS == 0) S.xg[u][j] = x64[col];
-            else S.xg[u][j] = x32[col];
+            if constexpr (SpmvStage<CLS, U, PM>::X32) S.xg[u][j] = x32[col];
+            else S.xg[u][j] = x64[col];
         }
     }
 }
@@ -439,8 +442,8 @@ __device__ __forceinline__ void spmv_load_gathers(const DevMatrix &A, SpmvStage<
 // Products of a batch whose values and gathers have arrived, added into q (inactive
 // entries add 0.0 to a lane-private dummy word; the mask is a multiplication so the
 // product and its loads are never moved into a branch).
-template <int CLS, int U, bool DYN>
-__device__ __forceinline__ void spmv_accumulate(const DevMatrix &A, const SpmvStage<CLS, U> &S, int row0, int sub,
+template <int CLS, int U, bool DYN, int PM>
+__device__ __forceinline__ void spmv_accumulate(const DevMatrix &A, const SpmvStage<CLS, U, PM> &S, int row0, int sub,
                                                 int lane, const float *__restrict__ x32, double *q, double *qdummy)
 {
 #pragma unroll
@@ -452,8 +455,9 @@ __device__ __forceinline__ void spmv_accumulate(const DevMatrix &A, const SpmvSt
             bool act = 4 * sub + j <= lastk;
             const int qi = rbase + ((S.id[u] >> (8 * j + 4)) & 15);
             double prod;
-            if constexpr (CLS == 0) prod = quad_value<CLS>(S.val[u], j) * S.xg[u][j];
-            else prod = (double)(quad_value<CLS>(S.val[u], j) * S.xg[u][j]);
+            if constexpr (CLS == 0) prod = quad_value<CLS>(S.val[u], j) * (double)S.xg[u][j];
+            else if constexpr (SpmvStage<CLS, U, PM>::X32) prod = (double)(quad_value<CLS>(S.val[u], j) * S.xg[u][j]);
+            else prod = (double)quad_value<CLS>(S.val[u], j) * S.xg[u][j];
             if constexpr (DYN) {
                 if (act && S.dc[u] > CLS) {
                     // lowered below the storage class: the value copy at that precision
@@ -474,22 +478,23 @@ __device__ __forceinline__ void spmv_accumulate(const DevMatrix &A, const SpmvSt
 // gathers issued earlier), b batch j+1 (headers, indices and values), c batch j+2
 // (headers).  Issues b's gathers, c's indices and values, accumulates a, then reloads
 // a's headers with batch j+3 (first chunk cb3).
-template <int CLS, int U, bool DYN>
-__device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> &a, SpmvStage<CLS, U> &b,
-                                          SpmvStage<CLS, U> &c, int cb3, int last, int row0, int sub, int lane,
+template <int CLS, int U, bool DYN, int PM>
+__device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U, PM> &a, SpmvStage<CLS, U, PM> &b,
+                                          SpmvStage<CLS, U, PM> &c, int cb3, int last, int row0, int sub, int lane,
                                           const double *__restrict__ x64, const float *__restrict__ x32, double *q,
                                           double *qdummy)
 {
-    spmv_load_gathers<CLS, U, DYN>(A, b, sub, x64, x32);
-    spmv_load_entries<CLS, U>(A, c, sub);
-    spmv_accumulate<CLS, U, DYN>(A, a, row0, sub, lane, x32, q, qdummy);
-    spmv_load_headers<CLS, U>(A, a, cb3, last);
+    spmv_load_gathers<CLS, U, DYN, PM>(A, b, sub, x64, x32);
+    spmv_load_entries<CLS, U, PM>(A, c, sub);
+    spmv_accumulate<CLS, U, DYN, PM>(A, a, row0, sub, lane, x32, q, qdummy);
+    spmv_load_headers<CLS, U, PM>(A, a, cb3, last);
 }
 
-// Class-CLS chunks [c0, c1) (headers sorted by class): q[row - row0] += a * x.
-//   CLS 0: binary64 values times x64, product in binary64.
-//   CLS 1/2/3: binary32/binary16/int8 values times x32, product in binary32, added in
-//   binary64.
+// Class-CLS chunks [c0, c1) (headers sorted by class): q[row - row0] += a * x, in the
+// arithmetic of product mode PM (sell.h):
+//   CLS 0: binary64 values times x64 (or, kPmLow32, the widened x32), product in binary64.
+//   CLS 1/2/3: binary32/binary16/int8 values times x32, product in binary32 (or,
+//   kPmWide64, the widened value times x64 in binary64), added in binary64.
 //   DYN: the chunk's class is lowered to its column tile's dynamic class (A.dvis) when
 //   that is lower, reading the value copy of that precision; dynamic class 4 skips it.
 // Four lanes share a chunk, each owning 4 consecutive entries: one 4-byte load of entry
@@ -502,7 +507,7 @@ __device__ __forceinline__ void spmv_step(const DevMatrix &A, SpmvStage<CLS, U> 
 // index, padded per-entry arrays), so the loop has no branch for the compiler to sink a
 // load into; the loop exit is a scalar test after each group of three batches (the last
 // batches of a group may lie past the end: clamped loads, every entry inactive).
-template <int CLS, int U, bool DYN>
+template <int CLS, int U, bool DYN, int PM = kPmTiles>
 __device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, long long c1, int row0,
                                            const double *__restrict__ x64, const float *__restrict__ x32,
                                            double *q, double *qdummy)
@@ -517,39 +522,40 @@ __device__ __forceinline__ void spmv_class(const DevMatrix &A, long long c0, lon
     const int last = (int)c1 - 1;
     int bw = (int)c0 + wv * 16 * U;
     if (bw > last) return;
-    SpmvStage<CLS, U> S0, S1, S2;
-    spmv_load_headers<CLS, U>(A, S0, bw + cs, last);
-    spmv_load_headers<CLS, U>(A, S1, bw + step + cs, last);
-    spmv_load_headers<CLS, U>(A, S2, bw + 2 * step + cs, last);
-    spmv_load_entries<CLS, U>(A, S0, sub);
-    spmv_load_entries<CLS, U>(A, S1, sub);
-    spmv_load_gathers<CLS, U, DYN>(A, S0, sub, x64, x32);
+    SpmvStage<CLS, U, PM> S0, S1, S2;
+    spmv_load_headers<CLS, U, PM>(A, S0, bw + cs, last);
+    spmv_load_headers<CLS, U, PM>(A, S1, bw + step + cs, last);
+    spmv_load_headers<CLS, U, PM>(A, S2, bw + 2 * step + cs, last);
+    spmv_load_entries<CLS, U, PM>(A, S0, sub);
+    spmv_load_entries<CLS, U, PM>(A, S1, sub);
+    spmv_load_gathers<CLS, U, DYN, PM>(A, S0, sub, x64, x32);
     do {
-        spmv_step<CLS, U, DYN>(A, S0, S1, S2, bw + 3 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
-        spmv_step<CLS, U, DYN>(A, S1, S2, S0, bw + 4 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
-        spmv_step<CLS, U, DYN>(A, S2, S0, S1, bw + 5 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN, PM>(A, S0, S1, S2, bw + 3 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN, PM>(A, S1, S2, S0, bw + 4 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
+        spmv_step<CLS, U, DYN, PM>(A, S2, S0, S1, bw + 5 * step + cs, last, row0, sub, lane, x64, x32, q, qdummy);
         bw += 3 * step;
     } while (bw <= last);
 }
 
 // q[row - row0] += (A x)[row] over one range's chunks; cr[0..4] are its class bounds.
 //   MIXED = false: every chunk as class 0 (binary64 values, x64).
-//   MIXED = true : each class with its own loop.
+//   MIXED = true : each class with its own loop, in the arithmetic of product mode PM.
 //   qdummy: 64 words of LDS that inactive lanes add 0.0 to (any LDS words whose value
 //   may not change, e.g. a reduction scratch area).
-template <bool MIXED, int U = 4, bool DYN = false>
+template <bool MIXED, int U = 4, bool DYN = false, int PM = kPmTiles>
 __device__ __forceinline__ void spmv_chunks(const DevMatrix &A, const long long *cr, int row0,
                                             const double *__restrict__ x64, const float *__restrict__ x32,
                                             double *q, double *qdummy)
 {
     constexpr int U0 = U > 1 ? U / 2 : 1;  // binary64 groups need twice the registers
+    constexpr int UL = PM == kPmWide64 ? U0 : U;  // so do binary64 gathers
     if constexpr (!MIXED) {
         spmv_class<0, U0, false>(A, cr[0], cr[4], row0, x64, x32, q, qdummy);
     } else {
-        spmv_class<0, U0, DYN>(A, cr[0], cr[1], row0, x64, x32, q, qdummy);
-        spmv_class<1, U, DYN>(A, cr[1], cr[2], row0, x64, x32, q, qdummy);
-        spmv_class<2, U, DYN>(A, cr[2], cr[3], row0, x64, x32, q, qdummy);
-        spmv_class<3, U, DYN>(A, cr[3], cr[4], row0, x64, x32, q, qdummy);
+        spmv_class<0, U0, DYN, PM>(A, cr[0], cr[1], row0, x64, x32, q, qdummy);
+        spmv_class<1, UL, DYN, PM>(A, cr[1], cr[2], row0, x64, x32, q, qdummy);
+        spmv_class<2, UL, DYN, PM>(A, cr[2], cr[3], row0, x64, x32, q, qdummy);
+        spmv_class<3, UL, DYN, PM>(A, cr[3], cr[4], row0, x64, x32, q, qdummy);
     }
 }
 

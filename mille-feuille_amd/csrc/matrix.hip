@@ -191,7 +191,6 @@ static mf_status build_layout(const TileStruct &ts, const double *val, const std
     L.max_offset = max_off;
     L.has_lowp = any_lowp && !v32.empty();
     L.has_copies = !v32.empty();
-    L.need_p64_mixed = L.tiles_by_class[MF_FP64] > 0;
     mf_status st;
     if ((st = upload((void **)&L.eidx, eid.data(), P, s)) != MF_SUCCESS) return st;
     if ((st = upload((void **)&L.v64, v64.data(), sizeof(double) * P, s)) != MF_SUCCESS) return st;

@@ -59,6 +59,24 @@ template <int S, bool NAR = false>
 constexpr int sell_e() { return S == 0 || S == 4 ? 2 : S == 3 ? (NAR ? 4 : 8) : (NAR ? 2 : 4); }
 template <int S> constexpr int sell_vbytes() { return S == 0 || S == 4 ? 8 : S == 1 ? 4 : S == 2 ? 2 : 1; }
 template <int S> constexpr int sell_cls() { return S == 4 ? 0 : S; }
+
+// Product modes of the tile-mixed SpMV (which copy of the gathered vector each class
+// multiplies; the sums are binary64 in every mode):
+//   kPmTiles    class 0: binary64 value x binary64 vector; classes 1-3: binary32 products
+//               of the value and the binary32 vector (mf_spmv's tile-class arithmetic)
+//   kPmLow32    every class multiplies the binary32 copy of the vector: class 0 in binary64
+//               (value x widened binary32 vector), classes 1-3 in binary32.  A row's
+//               product is then the product with one consistently rounded vector, which
+//               keeps CG's iteration count where mixing binary64 and binary32 copies in
+//               one row (kPmTiles) can add 10 % (shifted 2D Poisson, 12,544 rows: 389
+//               against 355 iterations)
+//   kPmWide64   every class multiplies the binary64 vector in binary64 (values widened
+//               exactly): storage-only precision lowering, the FP64 product of the stored
+//               values, for BiCGSTAB, whose recurrence binary32 products perturb
+enum { kPmTiles = 0, kPmLow32 = 1, kPmWide64 = 2 };
+// Whether class `cls` gathers the binary32 copy of the vector under product mode PM.
+template <int PM>
+__host__ __device__ constexpr bool pm_x32(int cls) { return cls == 0 ? PM == kPmLow32 : PM != kPmWide64; }
 inline int sell_e_host(int s, bool nar) { return s == 0 || s == 4 ? 2 : s == 3 ? (nar ? 4 : 8) : (nar ? 2 : 4); }
 inline int sell_vbytes_host(int s) { return s == 0 || s == 4 ? 8 : s == 1 ? 4 : s == 2 ? 2 : 1; }
 
@@ -132,54 +150,63 @@ __device__ __forceinline__ Words<NW> load_words(const void *base, size_t i)
     return r;
 }
 
-// Pipeline registers of one unit: column words, value words, gathered x.
-template <int S, bool NAR>
+// Pipeline registers of one unit: column words, value words, gathered x (binary32 or
+// binary64 copy as product mode PM has it for this stream's class).
+template <int S, bool NAR, int PM = kPmTiles>
 struct SellUnit {
     static constexpr int E = sell_e<S, NAR>();
     static constexpr int NWI = E / 2;                       // 2 B per column offset
     static constexpr int NWV = E * sell_vbytes<S>() / 4;
-    using XT = std::conditional_t<sell_cls<S>() == 0, double, float>;
+    static constexpr bool X32 = pm_x32<PM>(sell_cls<S>());
+    using XT = std::conditional_t<X32, float, double>;
     Words<NWI> d;
     Words<NWV> v;
     XT xg[E];
 };
 
 // ilane: the lane whose column words this lane reads (lane & the wave's lmask)
-template <int S, bool NAR>
-__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR> &U, int u, int lane, int ilane)
+template <int S, bool NAR, int PM>
+__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR, PM> &U, int u, int lane, int ilane)
 {
-    U.d = load_words<SellUnit<S, NAR>::NWI>(st.idx, (size_t)u * 64 + ilane);
-    U.v = load_words<SellUnit<S, NAR>::NWV>(st.val, (size_t)u * 64 + lane);
+    U.d = load_words<SellUnit<S, NAR, PM>::NWI>(st.idx, (size_t)u * 64 + ilane);
+    U.v = load_words<SellUnit<S, NAR, PM>::NWV>(st.val, (size_t)u * 64 + lane);
 }
 
-template <int S, bool NAR>
-__device__ __forceinline__ void sell_gather(SellUnit<S, NAR> &U, int row, const SellX &X)
+template <int S, bool NAR, int PM>
+__device__ __forceinline__ void sell_gather(SellUnit<S, NAR, PM> &U, int row, const SellX &X)
 {
 #pragma unroll
-    for (int j = 0; j < SellUnit<S, NAR>::E; j++) {
+    for (int j = 0; j < SellUnit<S, NAR, PM>::E; j++) {
         const int col = row + (int)(short)(U.d.w[j >> 1] >> (16 * (j & 1)));
-        if constexpr (sell_cls<S>() == 0) U.xg[j] = sell_buf_load_f64(X.r64, col, 0, 0, 0);
-        else U.xg[j] = sell_buf_load_f32(X.r32, col, 0, 0, 0);
+        if constexpr (SellUnit<S, NAR, PM>::X32) U.xg[j] = sell_buf_load_f32(X.r32, col, 0, 0, 0);
+        else U.xg[j] = sell_buf_load_f64(X.r64, col, 0, 0, 0);
     }
 }
 
-// Sum of the unit's products: binary64 for class 0, binary32 products added in binary64
-// otherwise (the tile classes' arithmetic of the chunk SpMV).
-template <int S, bool NAR>
-__device__ __forceinline__ double sell_products(const SellUnit<S, NAR> &U)
+// Sum of the unit's products in the arithmetic of product mode PM: class 0 in binary64;
+// classes 1-3 as binary32 products of the value and the binary32 vector, or (kPmWide64)
+// in binary64 with the value widened exactly; the products are added in binary64.
+template <int S, bool NAR, int PM>
+__device__ __forceinline__ double sell_products(const SellUnit<S, NAR, PM> &U)
 {
+    using UT = SellUnit<S, NAR, PM>;
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < SellUnit<S, NAR>::E; j++) {
+    for (int j = 0; j < UT::E; j++) {
         if constexpr (sell_cls<S>() == 0) {
-            s += __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]) * U.xg[j];
-        } else if constexpr (S == 1) {
-            s += (double)(__uint_as_float(U.v.w[j]) * U.xg[j]);
-        } else if constexpr (S == 2) {
-            const unsigned short h = (unsigned short)(U.v.w[j >> 1] >> (16 * (j & 1)));
-            s += (double)((float)__builtin_bit_cast(_Float16, h) * U.xg[j]);
+            s += __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]) * (double)U.xg[j];
         } else {
-            s += (double)((float)(signed char)(U.v.w[j >> 2] >> (8 * (j & 3))) * U.xg[j]);
+            float v;
+            if constexpr (S == 1) {
+                v = __uint_as_float(U.v.w[j]);
+            } else if constexpr (S == 2) {
+                const unsigned short h = (unsigned short)(U.v.w[j >> 1] >> (16 * (j & 1)));
+                v = (float)__builtin_bit_cast(_Float16, h);
+            } else {
+                v = (float)(signed char)(U.v.w[j >> 2] >> (8 * (j & 3)));
+            }
+            if constexpr (UT::X32) s += (double)(v * U.xg[j]);
+            else s += (double)v * U.xg[j];
         }
     }
     return s;
@@ -188,13 +215,13 @@ __device__ __forceinline__ double sell_products(const SellUnit<S, NAR> &U)
 // Column and value words of a stream's first two units for this wave, loaded ahead of the
 // SpMV that sums them (they do not depend on the vector): the SpMV then starts with only
 // the gathers to wait for.
-template <int S, bool NAR>
+template <int S, bool NAR, int PM = kPmTiles>
 struct SellLead {
-    SellUnit<S, NAR> u0, u1;
+    SellUnit<S, NAR, PM> u0, u1;
 };
 
-template <int S, int RPT, bool NAR>
-__device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, SellLead<S, NAR> &L)
+template <int S, int RPT, bool NAR, int PM>
+__device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, SellLead<S, NAR, PM> &L)
 {
     const int lane = threadIdx.x & 63;
     const int *sb = st.soff + (size_t)slot * RPT;
@@ -202,8 +229,8 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
     const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
     if (u0 > last) return;  // the wave has no unit in this stream (the SpMV skips it too)
     const int ilane = lane & __builtin_amdgcn_readfirstlane(st.lmask[slot]);
-    sell_load<S, NAR>(st, L.u0, u0, lane, ilane);
-    sell_load<S, NAR>(st, L.u1, min(u0 + 1, last), lane, ilane);
+    sell_load<S, NAR, PM>(st, L.u0, u0, lane, ilane);
+    sell_load<S, NAR, PM>(st, L.u1, min(u0 + 1, last), lane, ilane);
 }
 
 // q[k] += (A x)[row0 + BLK k + 64 v + lane] over stream S's entries.  Column words are
@@ -223,10 +250,10 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
 // q's order.  Unit bounds are wave-uniform (scalar loads).
 // LEAD: the first two units were loaded by sell_prefetch into *lead (whose registers the
 // pipeline then reuses: prefetch again before the next use).
-template <int S, int RPT, int BLK, bool NAR, bool LEAD = false>
+template <int S, int RPT, int BLK, bool NAR, int PM = kPmTiles, bool LEAD = false>
 __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
                                             const float *__restrict__ x32, double (&q)[RPT],
-                                            SellLead<S, NAR> *lead = nullptr)
+                                            SellLead<S, NAR, PM> *lead = nullptr)
 {
     const int lane = threadIdx.x & 63;
     const int *sb = st.soff + (size_t)slot * RPT;
@@ -251,25 +278,26 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
         rot++;
     };
     while (rot < RPT - 1 && e[0] <= u0) close_slice();  // leading empty slices
-    SellUnit<S, NAR> l0, l1, S2;
-    SellUnit<S, NAR> &S0 = LEAD ? lead->u0 : l0;
-    SellUnit<S, NAR> &S1 = LEAD ? lead->u1 : l1;
+    using UT = SellUnit<S, NAR, PM>;
+    UT l0, l1, S2;
+    UT &S0 = LEAD ? lead->u0 : l0;
+    UT &S1 = LEAD ? lead->u1 : l1;
     double acc = 0.0;
     int i = u0;
     if constexpr (!LEAD) {
-        sell_load<S, NAR>(st, S0, i, lane, ilane);
-        sell_load<S, NAR>(st, S1, min(i + 1, last), lane, ilane);
+        sell_load<S, NAR, PM>(st, S0, i, lane, ilane);
+        sell_load<S, NAR, PM>(st, S1, min(i + 1, last), lane, ilane);
     }
-    sell_gather<S, NAR>(S0, rowb, X);
+    sell_gather<S, NAR, PM>(S0, rowb, X);
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
     // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
-    auto step = [&](SellUnit<S, NAR> &a, SellUnit<S, NAR> &nb1, SellUnit<S, NAR> &nb2) {
-        sell_load<S, NAR>(st, nb2, min(i + 2, last), lane, ilane);
+    auto step = [&](UT &a, UT &nb1, UT &nb2) {
+        sell_load<S, NAR, PM>(st, nb2, min(i + 2, last), lane, ilane);
         __builtin_amdgcn_sched_barrier(0);
-        sell_gather<S, NAR>(nb1, rowb, X);
+        sell_gather<S, NAR, PM>(nb1, rowb, X);
         __builtin_amdgcn_sched_barrier(0);
-        const double pr = sell_products<S, NAR>(a);
+        const double pr = sell_products<S, NAR, PM>(a);
         acc += i <= last ? pr : 0.0;
         const bool cl = i + 1 == e[0];  // unit i ends the current slice
         const double q0 = q[0] + acc;
@@ -327,42 +355,42 @@ struct RowVec {
 };
 
 // q[k] = (A x) of this thread's rows; rowb is the thread's slot-0 row (the base of the
-// column offsets).  MIXED: streams 0..3 (tile classes, binary32 x for classes 1..3);
-// otherwise stream 4 (binary64 values and x).
+// column offsets).  MIXED: streams 0..3 (tile classes, in the arithmetic of product mode
+// PM); otherwise stream 4 (binary64 values and x).
 // The stream whose first units the variant-4 CG kernels prefetch: int8 (the class the
 // mixed SpMV sums first) or binary64.
 template <bool MIXED> constexpr int sell_lead_stream() { return MIXED ? 3 : kSellAll64; }
 
-template <bool MIXED, int RPT, int BLK, bool NAR = false>
+template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles>
 __device__ __forceinline__ void spmv_sell_lead(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                                const float *__restrict__ x32, double (&q)[RPT],
-                                               SellLead<sell_lead_stream<MIXED>(), NAR> &lead)
+                                               SellLead<sell_lead_stream<MIXED>(), NAR, PM> &lead)
 {
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {
-        sell_stream<3, RPT, BLK, NAR, true>(D.s[3], slot, rowb, x64, x32, q, &lead);
-        sell_stream<0, RPT, BLK, NAR>(D.s[0], slot, rowb, x64, x32, q);
-        sell_stream<1, RPT, BLK, NAR>(D.s[1], slot, rowb, x64, x32, q);
-        sell_stream<2, RPT, BLK, NAR>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK, NAR, PM, true>(D.s[3], slot, rowb, x64, x32, q, &lead);
+        sell_stream<0, RPT, BLK, NAR, PM>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR, PM>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR, PM>(D.s[2], slot, rowb, x64, x32, q);
     } else {
-        sell_stream<4, RPT, BLK, NAR, true>(D.s[4], slot, rowb, x64, x32, q, &lead);
+        sell_stream<4, RPT, BLK, NAR, PM, true>(D.s[4], slot, rowb, x64, x32, q, &lead);
     }
 }
 
-template <bool MIXED, int RPT, int BLK, bool NAR = false>
+template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles>
 __device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                           const float *__restrict__ x32, double (&q)[RPT])
 {
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {
-        sell_stream<0, RPT, BLK, NAR>(D.s[0], slot, rowb, x64, x32, q);
-        sell_stream<1, RPT, BLK, NAR>(D.s[1], slot, rowb, x64, x32, q);
-        sell_stream<2, RPT, BLK, NAR>(D.s[2], slot, rowb, x64, x32, q);
-        sell_stream<3, RPT, BLK, NAR>(D.s[3], slot, rowb, x64, x32, q);
+        sell_stream<0, RPT, BLK, NAR, PM>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR, PM>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR, PM>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK, NAR, PM>(D.s[3], slot, rowb, x64, x32, q);
     } else {
-        sell_stream<4, RPT, BLK, NAR>(D.s[4], slot, rowb, x64, x32, q);
+        sell_stream<4, RPT, BLK, NAR, PM>(D.s[4], slot, rowb, x64, x32, q);
     }
 }
 

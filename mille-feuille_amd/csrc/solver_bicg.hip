@@ -71,17 +71,22 @@ __device__ __forceinline__ void publish(const BicgArgs &a, Rows<RPT, BLK> &R, co
 }
 
 // q <- A v (tile classes when MIXED) for the vector published in x64/x32: sliced-ELL
-// streams into registers, or tile chunks into the workgroup's LDS slice.
+// streams into registers, or tile chunks into the workgroup's LDS slice.  The tile-mixed
+// product widens every stored value and multiplies the binary64 vector in binary64
+// (kPmWide64, sell.h): BiCGSTAB's recurrence does not tolerate binary32 products of its
+// search directions (on 3D convection-diffusion 128^3 they cost 8 % more iterations, and
+// 2D Poisson 512^2 diverged), so the lowered tiles save bytes, not arithmetic.
 template <bool MIXED, int U, int SELL, int RPT, int BLK>
 __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr, int slot, Rows<RPT, BLK> &R,
                                           const double *x64, const float *x32, QVec<RPT, (SELL != 0)> &q, double *scratch)
 {
+    constexpr int PM = MIXED ? kPmWide64 : kPmTiles;
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4)>(a.S, slot, R.base(), x64, x32, q.reg);
+        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4), PM>(a.S, slot, R.base(), x64, x32, q.reg);
     } else {
         zero_q<RPT, BLK>(q.lds, R);
         __syncthreads();
-        spmv_chunks<MIXED, U>(a.A, cr, R.row0, x64, x32, q.lds, scratch);
+        spmv_chunks<MIXED, U, false, PM>(a.A, cr, R.row0, x64, x32, q.lds, scratch);
         __syncthreads();
     }
 }
@@ -489,9 +494,14 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     a.maxit = o->max_iter;
     a.fixed = o->fixed_iterations;
     a.use_x0 = o->use_x0;
-    a.need64 = (!mixed_arith || A->L.need_p64_mixed) ? 1 : 0;
-    a.need32 = mixed_arith ? 1 : 0;
+    a.need64 = 1;  // every product takes the binary64 p and s (kPmWide64)
+    a.need32 = 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
+    // Lossless classes store every value exactly, and the tile-mixed product widens them
+    // and multiplies in binary64 (kPmWide64): the product is the FP64 one, so the solve runs
+    // the FP64 recurrence itself, without residual replacements or restarts (each would
+    // perturb the recurrence: 2D Poisson 512^2 took 1,063 against 1,037 iterations with them).
+    if (mixed_arith && A->class_policy == MF_CLASS_LOSSLESS) a.max_repl = 0;
     a.S = dsell;
     a.nb.flags = A->ws.flags;
     a.nb.dep_ptr = P->dep_ptr;

@@ -295,13 +295,13 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     a.maxit = o->max_iter;
     a.fixed = o->fixed_iterations;
     a.use_x0 = o->use_x0;
-    a.need64 = (!mixed_arith || A->L.need_p64_mixed) ? 1 : 0;
+    // tile-mixed products take the binary32 p in every class (kPmLow32)
+    a.need64 = mixed_arith ? 0 : 1;
     a.need32 = mixed_arith ? 1 : 0;
     a.max_repl = o->max_replacements > 0 ? o->max_replacements : 64;
     a.dvis = w.dvis;
     a.dyn_mode = dyn ? (o->dynamic_precision == 2 ? 2 : 1) : 0;
     a.A.dvis = dyn ? w.dvis : nullptr;
-    if (dyn) a.need64 = 1;  // dynamic classes may keep any tile at binary64
     a.S = dsell;
     a.nb.flags = w.flags;
     a.nb.dep_ptr = P->dep_ptr;

@@ -67,7 +67,6 @@ struct DeviceLayout {
     int max_offset = 0;             // max |col - row| over the entries
     bool has_lowp = false;
     bool has_copies = false;        // v32/v16/v8 hold every entry (lossy where the class is higher)
-    bool need_p64_mixed = false;
     int64_t tiles_by_class[4] = {0, 0, 0, 0};
     int64_t nnz_by_class[4] = {0, 0, 0, 0};
     double alg_bytes_f64 = 0, alg_bytes_mixed = 0, device_bytes = 0;

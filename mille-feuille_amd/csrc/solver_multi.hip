@@ -20,7 +20,9 @@ namespace mfk {
 
 constexpr int kDotBlocks = 512;
 
-template <bool MIXED>
+// PM: product mode of the tile-mixed SpMV (sell.h): kPmLow32 for CG, kPmWide64 for
+// BiCGSTAB, as in the persistent kernels.
+template <bool MIXED, int PM>
 __global__ __launch_bounds__(1024) void mk_spmv(DevMatrix A, const double *__restrict__ x64,
                                                 const float *__restrict__ x32, double *__restrict__ y)
 {
@@ -31,7 +33,7 @@ __global__ __launch_bounds__(1024) void mk_spmv(DevMatrix A, const double *__res
     const int row0 = tr0 * kTile, row1 = min(tr1 * kTile, A.n);
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) q[i] = 0.0;
     __syncthreads();
-    spmv_chunks<MIXED, 2>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
+    spmv_chunks<MIXED, 2, false, PM>(A, A.wg_chunk + 4 * (size_t)w, row0, x64, x32, q, qdummy);
     __syncthreads();
     for (int i = threadIdx.x; i < row1 - row0; i += blockDim.x) y[row0 + i] = q[i];
 }
@@ -190,12 +192,15 @@ struct MkCtx {
         bufs.push_back(p);
         return (T *)p;
     }
-    void spmv(const double *x64, const float *x32, double *y, bool use_mixed)
+    // use_mixed: tile classes with product mode pm (kPmLow32 CG, kPmWide64 BiCGSTAB)
+    void spmv(const double *x64, const float *x32, double *y, bool use_mixed, int pm = mfk::kPmLow32)
     {
-        if (use_mixed)
-            hipLaunchKernelGGL(mfk::mk_spmv<true>, dim3(G), dim3(1024), lds, s, dm, x64, x32, y);
+        if (use_mixed && pm == mfk::kPmWide64)
+            hipLaunchKernelGGL((mfk::mk_spmv<true, mfk::kPmWide64>), dim3(G), dim3(1024), lds, s, dm, x64, x32, y);
+        else if (use_mixed)
+            hipLaunchKernelGGL((mfk::mk_spmv<true, mfk::kPmLow32>), dim3(G), dim3(1024), lds, s, dm, x64, x32, y);
         else
-            hipLaunchKernelGGL(mfk::mk_spmv<false>, dim3(G), dim3(1024), lds, s, dm, x64, x32, y);
+            hipLaunchKernelGGL((mfk::mk_spmv<false, mfk::kPmTiles>), dim3(G), dim3(1024), lds, s, dm, x64, x32, y);
     }
     void dot(const double *a, const double *b, int slot)
     {
@@ -231,7 +236,8 @@ mf_status mk_setup(mf_matrix *A, const mf_solve_options *o, hipStream_t s, MkCtx
     c.dm = A->L.dev(*c.P);
     c.lds = (size_t)std::max(c.P->rows_cap, 1) * 8;
     if (c.lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
-    for (auto k : {(const void *)mfk::mk_spmv<true>, (const void *)mfk::mk_spmv<false>})
+    for (auto k : {(const void *)mfk::mk_spmv<true, mfk::kPmLow32>, (const void *)mfk::mk_spmv<true, mfk::kPmWide64>,
+                   (const void *)mfk::mk_spmv<false, mfk::kPmTiles>})
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds);
     c.partial = c.alloc<double>(mfk::kDotBlocks);
     c.sc = c.alloc<double>(16);
@@ -389,10 +395,10 @@ mf_status bicg_multikernel(mf_matrix *A, const double *b, double *x, const mf_so
             status = o->fixed_iterations ? MF_SUCCESS : MF_NOT_CONVERGED;
             break;
         }
-        c.spmv(p, p32, v, c.mixed);
+        c.spmv(p, p32, v, c.mixed, mfk::kPmWide64);
         c.dot(rh, v, 1);
         hipLaunchKernelGGL(mfk::mk_bicg_s, vg, dim3(256), 0, s, c.sc, r, v, sv, s32, n);
-        c.spmv(sv, s32, t, c.mixed);
+        c.spmv(sv, s32, t, c.mixed, mfk::kPmWide64);
         c.dot(t, sv, 2);
         c.dot(t, t, 3);
         hipLaunchKernelGGL(mfk::mk_bicg_xr, vg, dim3(256), 0, s, c.sc, x, r, p, sv, t, n);

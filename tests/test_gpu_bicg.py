@@ -66,19 +66,29 @@ def test_bicgstab_mixed_reaches_fp64_tolerance(mf, orc, torch, name, layout, var
     A.close()
 
 
+@pytest.mark.parametrize("nx", [256, 512])
 @pytest.mark.parametrize("layout", ["auto", "chunks"])
-def test_bicgstab_mixed_restarts_after_breakdown(mf, orc, torch, layout):
-    """2D Poisson 256 x 256 (int8 tiles): with binary32 products the textbook recurrence
-    reaches a zero rho or omega before 1e-10; the tile-mixed solve restarts from the binary64
-    residual (rhat = p = r) and still reaches the FP64 tolerance."""
-    rp, ci, va = mf.workloads.poisson2d(256)
+def test_bicgstab_mixed_follows_fp64(mf, orc, torch, layout, nx):
+    """2D Poisson 256^2 and 512^2 (int8 tiles).  With binary32 products of p and s the
+    recurrence needed 4x the iterations at 256^2 and diverged at 512^2; with the stored
+    values widened and multiplied by the binary64 vectors the tile-mixed solve is the FP64
+    recurrence on the same values, summed in another order (class streams).  On these
+    symmetric systems BiCGSTAB's count is that sensitive to summation order alone: the
+    oracle takes 515 iterations at 256^2 and 508 with every row's entries reversed (SciPy's
+    bicgstab 548); at 512^2 1,099, 887 and 1,018.  So the counts are held to 20 % of the
+    FP64 solve's (the 3D convection-diffusion system holds 2 %), with both at 1e-10."""
+    rp, ci, va = mf.workloads.poisson2d(nx)
     b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
     A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
-    print(f"bicg restart poisson2d 256 {layout}", flush=True)
-    x, info = A.bicgstab(b, tol=1e-10, precision="mixed", layout=layout, max_iter=20000)
-    print(info, flush=True)
-    assert info["status"] == 0 and info["replacements"] >= 1, info
-    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    print(f"bicg poisson2d {nx} {layout}", flush=True)
+    x64, i64 = A.bicgstab(b, tol=1e-10, precision="fp64", layout=layout, max_iter=20000)
+    xm, im = A.bicgstab(b, tol=1e-10, precision="mixed", layout=layout, max_iter=20000)
+    print(f"  fp64 {i64['iterations']} mixed {im['iterations']} (repl {im['replacements']})", flush=True)
+    assert i64["status"] == 0 and im["status"] == 0, (i64, im)
+    assert im["precision"] == 1
+    assert abs(im["iterations"] - i64["iterations"]) <= 0.2 * i64["iterations"]
+    assert orc.true_relres(rp, ci, va, x64.cpu().numpy(), b) <= 1.2e-10
+    assert orc.true_relres(rp, ci, va, xm.cpu().numpy(), b) <= 1.2e-10
     A.close()
 
 

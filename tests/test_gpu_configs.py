@@ -88,8 +88,8 @@ def test_cfg2b_convdiff128_bicgstab(mf, orc, torch):
     assert _within(i64["iterations"], ref["iterations"])
     assert _host_relres(rp, ci, va, x64.cpu().numpy(), b) <= 1.01e-10
     assert _host_relres(rp, ci, va, xm.cpu().numpy(), b) <= 1.01e-10
-    # binary32 products of the published p and s perturb BiCGSTAB more than CG
-    assert im["iterations"] <= 1.10 * i64["iterations"]
+    # the tile-mixed BiCGSTAB multiplies the widened stored values by the binary64 p and s
+    assert _within(im["iterations"], i64["iterations"])
     A.close()
 
 
