@@ -4,7 +4,8 @@
  * Plain C types only: pointers, sizes, status codes.  Device pointers are
  * HIP device addresses; `stream` arguments are hipStream_t passed as void*
  * (NULL = the default stream).  One mf_matrix per host thread at a time; no
- * global mutable state is shared between handles.
+ * global mutable state is shared between handles, except the scratch array of
+ * mf_cg_solve_noalloc (one per device).
  *
  * Reference interfaces each group replaces are cited inline
  * (file:line in zhaijiaqi/Mille-feuille).
@@ -292,6 +293,23 @@ mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
  * main-bicg-mixed.cu:1355). */
 mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                             mf_solve_info *info, void *stream);
+/* mf_cg_solve without any device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The publish buffers of p, the grid-reduction
+ * records, flags and the info block come from one static device array of the library
+ * (about 25.4 MB per device that runs the library, held until the process ends), so:
+ *   - n_rows <= 2,097,152 and at most 512 workgroups (else MF_ERR_TOO_LARGE);
+ *   - the matrix's launch plan for these options (its partition for the grid and, for
+ *     sliced-ELL solves, the SpMV streams) must exist: an earlier mf_cg_solve with the
+ *     same options on this handle builds it; otherwise MF_ERR_UNSUPPORTED;
+ *   - variant 3 (pipelined) is refused (MF_ERR_UNSUPPORTED) and info->time_ms is 0
+ *     (no timing events);
+ *   - every call on one device shares the array: calls that overlap on a device race
+ *     (serialise them, e.g. on one stream).  Each launch clears its flags, records and
+ *     info block first, with a kernel.
+ * Results are those of mf_cg_solve with the same options (same kernel and grid). */
+mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
+                              mf_solve_info *info, void *stream);
 /* Autotuner of the persistent CG launch: times short fixed-iteration solves over
  * candidate workgroup counts and the four variants, caches the fastest for this matrix
  * and precision (used by later solves with grid = 0, variant = 0). */

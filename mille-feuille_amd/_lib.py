@@ -134,6 +134,8 @@ _SIGS = {
     "mf_dyn_classify": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp]),
     "mf_pattern_outer": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, vp, vp]),
     "mf_cg_solve": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo), vp]),
+    "mf_cg_solve_noalloc": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo),
+                                           vp]),
     "mf_bicgstab_solve": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo), vp]),
     "mf_cg_autotune": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, i32p, i32p, f64p, vp]),
     "mf_cg_solve_host": (ctypes.c_int, [vp, f64p, f64p, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveInfo)]),

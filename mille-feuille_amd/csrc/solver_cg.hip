@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "device_common.h"
 #include "mf_internal.h"
@@ -418,3 +419,77 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
     return MF_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------------
+// mf_cg_solve_noalloc: mf_cg_solve on a fixed device-resident scratch array instead of
+// the matrix's workspace, for callers that cannot allocate device memory.
+// ---------------------------------------------------------------------------------
+namespace mf {
+
+constexpr long long kNaMaxRows = 1LL << 21;  // 2,097,152 rows (3D Poisson 128^3)
+constexpr int kNaMaxGrid = 512;
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+// scratch layout: p64, p32, grid-reduction records, publish flags, control words, info
+// block, dynamic classes (each 256-byte aligned)
+constexpr size_t kNaP64 = 0;
+constexpr size_t kNaP32 = kNaP64 + na_round(8 * (size_t)(kNaMaxRows + 64));
+constexpr size_t kNaRecs = kNaP32 + na_round(4 * (size_t)(kNaMaxRows + 64));
+constexpr size_t kNaFlags = kNaRecs + na_round(8 * (size_t)mfk::kRecWordsPerMember * kNaMaxGrid);
+constexpr size_t kNaCtl = kNaFlags + na_round(4 * (size_t)mfk::kFlagStride * kNaMaxGrid);
+constexpr size_t kNaInfo = kNaCtl + 256;
+constexpr size_t kNaDvis = kNaInfo + 256;
+constexpr size_t kNaBytes = kNaDvis + na_round((size_t)kNaMaxRows / 16 + 64);  // about 25.4 MB
+
+}  // namespace mf
+
+// One copy per device, held from the first use of the library's code object on that device
+// until the process ends.  Host code never uses this name as an address: it takes each
+// device's address with hipGetSymbolAddress (na_scratch).
+static __device__ __align__(256) unsigned char scratch[mf::kNaBytes];
+
+namespace mf {
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mutex;
+static unsigned char *g_na_ptr[kNaMaxDevices];  // device address of `scratch` per device
+
+// Device address of `scratch` on `device` (current), looked up once per device.
+static mf_status na_scratch(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kNaMaxDevices) return MF_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MF_ERR_HIP;
+        g_na_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_na_ptr[device];
+    return MF_SUCCESS;
+}
+
+}  // namespace mf
+
+extern "C" mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
+                                         mf_solve_info *info, void *stream)
+{
+    if (!A || !d_b || !d_x || !aligned16(d_b) || !aligned16(d_x)) return MF_ERR_INVALID;
+    if (A->L.n_rows != A->L.n_cols) return MF_ERR_INVALID;
+    if (A->L.n_rows > kNaMaxRows) return MF_ERR_TOO_LARGE;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    mf_solve_options o;
+    mf_solve_options_default(&o);
+    if (opt) o = *opt;
+    if (o.transpose) return MF_ERR_UNSUPPORTED;
+    unsigned char *base = nullptr;
+    mf_status st = na_scratch(A->device, &base);
+    if (st != MF_SUCCESS) return st;
+    WsView w;
+    w.p64 = (double *)(base + kNaP64);
+    w.p32 = (float *)(base + kNaP32);
+    w.recs = (unsigned long long *)(base + kNaRecs);
+    w.flags = (unsigned *)(base + kNaFlags);
+    w.ctl = (unsigned *)(base + kNaCtl);
+    w.info = (double *)(base + kNaInfo);
+    w.dvis = (uint8_t *)(base + kNaDvis);
+    w.cap_g = kNaMaxGrid;
+    return cg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream, &w);
+}
