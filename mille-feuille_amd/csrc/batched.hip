@@ -69,6 +69,7 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
     gsy.gs.ctl = a.ctl;
     gsy.gs.timeout = a.timeout;
     gsy.gs.G = W;
+    gsy.gs.acq = 1;  // tile-chunk gathers are plain loads
     gsy.wg = member;
     gsy.epoch = 0;
     BlockOps bsy;

@@ -46,6 +46,11 @@ struct GridSync {
     unsigned *ctl;              // ctl[0] abort flag
     long long timeout;          // cycles of s_memtime a wait may last
     int G;
+    // 1: after a synchronisation that publishes data, acquire at agent scope (invalidate
+    // the L2 lines the following plain loads could find stale); 0: every load of data
+    // another workgroup published is itself an agent-scope load (the sliced-ELL gathers),
+    // so the L2 keeps the matrix stream from one iteration to the next
+    int acq;
 };
 
 __device__ __forceinline__ void wait_own_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
@@ -206,7 +211,7 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
             for (int i = 0; i < nwv; i++) m += s_part[threadIdx.x * 16 + i];
             s_bc[threadIdx.x] = m;
         }
-        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (visibility && gs.acq && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = s_bc[k];
@@ -253,7 +258,7 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
             s_bc[threadIdx.x] = t;
         }
         // after the broadcast (off the other members' critical path): drop stale L2 lines
-        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (visibility && gs.acq && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     } else {
         if (threadIdx.x < K) {
             double d = 0.0;
@@ -261,7 +266,7 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
             if (!poll_record(gs, rec_at(gs, epoch, threadIdx.x, G), epoch, t0, d, true, &ab) || ab) s_fail = 1;
             s_bc[threadIdx.x] = d;
         }
-        if (visibility && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (visibility && gs.acq && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 #pragma unroll
@@ -296,7 +301,8 @@ __device__ __forceinline__ void nb_signal(const NbSync &nb, int wg, unsigned epo
 }
 
 // Wave 0 polls the flags (one dependency per lane at a time); false when the watchdog
-// fired or the abort flag is set (which it then sets for everyone).  The workgroup
+// fired or the abort flag is set (which it then sets for everyone).  Then, unless the
+// gathers are agent-scope loads themselves (gs.acq = 0), an agent-scope acquire.  The workgroup
 // barrier at the end publishes the result; s_nbfail is next written by the following
 // call, after the barriers of the SpMV and grid reduction in between.
 __device__ __forceinline__ bool nb_wait(const GridSync &gs, const NbSync &nb, int wg, unsigned epoch)
@@ -322,7 +328,7 @@ __device__ __forceinline__ bool nb_wait(const GridSync &gs, const NbSync &nb, in
         }
         const unsigned long long bad = __ballot(!ok);
         if (threadIdx.x == 0) s_nbfail = bad != 0ull;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (gs.acq) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     return s_nbfail == 0;

@@ -86,7 +86,11 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 // Gathers of x go through buffer loads in index mode: the load takes the element index in
 // one VGPR (no 64-bit address arithmetic), and the hardware checks it against n (an index
-// outside [0, n) reads 0, never memory; the layouts only store columns inside).
+// outside [0, n) reads 0, never memory; the layouts only store columns inside).  They are
+// agent-scope loads (cache policy sc1): they see what other workgroups published with
+// write-through stores without an L2 invalidation before the SpMV, so the matrix stream
+// stays in L2 across iterations (3D Poisson 128^3, tile-mixed: SpMV 6.8 -> 5.5 us).
+constexpr int kSellGatherPolicy = 16;  // sc1
 __device__ float sell_buf_load_f32(i32x4 rsrc, int vindex, int voffset, int soffset, int aux)
     __asm("llvm.amdgcn.struct.buffer.load.f32");
 __device__ double sell_buf_load_f64(i32x4 rsrc, int vindex, int voffset, int soffset, int aux)
@@ -178,8 +182,8 @@ __device__ __forceinline__ void sell_gather(SellUnit<S, NAR, PM> &U, int row, co
 #pragma unroll
     for (int j = 0; j < SellUnit<S, NAR, PM>::E; j++) {
         const int col = row + (int)(short)(U.d.w[j >> 1] >> (16 * (j & 1)));
-        if constexpr (SellUnit<S, NAR, PM>::X32) U.xg[j] = sell_buf_load_f32(X.r32, col, 0, 0, 0);
-        else U.xg[j] = sell_buf_load_f64(X.r64, col, 0, 0, 0);
+        if constexpr (SellUnit<S, NAR, PM>::X32) U.xg[j] = sell_buf_load_f32(X.r32, col, 0, 0, kSellGatherPolicy);
+        else U.xg[j] = sell_buf_load_f64(X.r64, col, 0, 0, kSellGatherPolicy);
     }
 }
 
