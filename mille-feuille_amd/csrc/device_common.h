@@ -130,19 +130,27 @@ __device__ __forceinline__ void put_record(unsigned long long *rec, double v, un
     __hip_atomic_store(rec + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Records of one synchronisation group of G workgroups: for each epoch parity and
-// reduced value k, G member records (index = member) and one broadcast record (index G),
-// each 2 x u64 {lo32 | epoch << 32, hi32 | epoch << 32}.  A group needs rec_words(G)
-// u64; kRecWordsPerMember * G always suffices.
+// Records of one synchronisation group of G workgroups: for each epoch parity, G member
+// blocks (index = member) and one broadcast block (index G) of kMaxRed records, each
+// 2 x u64 {lo32 | epoch << 32, hi32 | epoch << 32}.  A member's records sit side by side
+// (64 bytes), so a cache line takes the writes of at most two members: with the records of
+// one value k of all members packed together (eight members per line), the gather of a
+// reduction at 256 workgroups took about 1 us longer.  A group needs rec_words(G) u64;
+// kRecWordsPerMember * G always suffices.
 constexpr int kRecWordsPerMember = 4 * kMaxRed * 2;
 __host__ __device__ constexpr long long rec_words(int G) { return 2LL * kMaxRed * (G + 1) * 2; }
 
 __device__ __forceinline__ unsigned long long *rec_at(const GridSync &gs, unsigned epoch, int k, int i)
 {
-    return gs.recs + (((size_t)(epoch & 1) * kMaxRed + k) * (size_t)(gs.G + 1) + i) * 2;
+    return gs.recs + (((size_t)(epoch & 1) * (size_t)(gs.G + 1) + i) * kMaxRed + k) * 2;
 }
 
 constexpr unsigned kAbortTag = 0x80000000u;  // broadcast tag bit: the group aborts at this sync
+
+// Polls between two reads of the shared abort flag: every waiting thread of the grid
+// reading that one word on each poll made it the hottest line of a reduction (at 256
+// workgroups about 1,000 root threads and 1,000 broadcast pollers).
+constexpr int kCtlPollStride = 32;
 
 // Waits for the record to carry `epoch` (or, with accept_abort, epoch | kAbortTag, which
 // sets *aborted).  False when the watchdog fired or the abort flag is set; then the abort
@@ -151,7 +159,7 @@ __device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned l
                                             long long t0, double &out, bool accept_abort = false,
                                             bool *aborted = nullptr)
 {
-    for (;;) {
+    for (int n = 1;; n++) {
         const unsigned long long lo = __hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long hi = __hip_atomic_load(rec + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned tl = (unsigned)(lo >> 32), th = (unsigned)(hi >> 32);
@@ -160,8 +168,8 @@ __device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned l
             if (aborted) *aborted = tl != epoch;
             return true;
         }
-        if (__hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-            __builtin_readcyclecounter() - t0 > gs.timeout) {
+        if (__builtin_readcyclecounter() - t0 > gs.timeout ||
+            (n % kCtlPollStride == 0 && __hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
             __hip_atomic_store(gs.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -233,10 +241,11 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
         double acc[K];
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] = 0.0;
+        // consecutive threads read a member's K records (adjacent words)
         for (int i = threadIdx.x; i < K * G; i += blockDim.x) {
-            const int k = i / G;
+            const int k = i % K;
             double d;
-            if (!poll_record(gs, rec_at(gs, epoch, k, i - k * G), epoch, t0, d)) {
+            if (!poll_record(gs, rec_at(gs, epoch, k, i / K), epoch, t0, d)) {
                 s_fail = 1;
                 break;
             }
@@ -314,11 +323,12 @@ __device__ __forceinline__ bool nb_wait(const GridSync &gs, const NbSync &nb, in
         const long long t0 = __builtin_readcyclecounter();
         for (int i = d0 + (int)threadIdx.x; i < d1 && ok; i += 64) {
             const unsigned *f = &nb.flags[(size_t)nb.dep_idx[i] * kFlagStride];
-            for (;;) {
+            for (int n = 1;; n++) {
                 const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if ((int)(v - epoch) >= 0) break;
-                if (__hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                    __builtin_readcyclecounter() - t0 > gs.timeout) {
+                if (__builtin_readcyclecounter() - t0 > gs.timeout ||
+                    (n % kCtlPollStride == 0 &&
+                     __hip_atomic_load(gs.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
                     __hip_atomic_store(gs.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = false;
                     break;
