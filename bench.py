@@ -1,11 +1,17 @@
 """Benchmark: single-kernel CG to a 1e-10 relative residual on the 3D 7-point Poisson
 problem on a 128^3 grid (n = 2,097,152, nnz = 14,581,760), tile-mixed vs all-FP64,
-plus BiCGSTAB on the 128^3 convection-diffusion problem.
+plus BiCGSTAB on the 128^3 convection-diffusion problem and the other benchmark
+workloads: plain FP64 CG on 2D Poisson 64 x 64, 512 batched SPD systems in one
+hipGraph launch, and the differentiable 64^3 variable-coefficient solve (forward +
+adjoint).  --extras adds the irregular SPD system at n = 10^6.
 
 One "step" is one complete solve of the tile-mixed CG (inputs resident in HBM, the
 matrix already in the tiled format).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-bicg]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extras]
+                  [--workload all|cg|bicg|poisson64|batched|autograd|irregular]
+
+--workload runs one workload only (for rocprofv3 runs that attribute counters to it).
 """
 from __future__ import annotations
 
@@ -36,18 +42,16 @@ def cpu_model():
     return platform.processor()
 
 
-def run_extras(mf, torch, dev, log):
-    """Batched CG over 512 SPD systems (one persistent hipGraph launch vs 512 single-system
-    launches), the differentiable solve on 64^3 variable-coefficient diffusion, the
-    irregular SPD system at n = 10^6, and plain FP64 CG on 2D Poisson 64 x 64."""
-    out = {}
+def run_batched(mf, torch, dev, log):
+    """512 SPD systems (log-uniform n in [1k, 64k], alternating 2D/3D Poisson with a diagonal
+    shift, seed 3) in one persistent hipGraph launch, against 512 single-system launches of
+    the same solver."""
     systems = mf.workloads.batched_poisson(512, seed=3)
     B = mf.Batch([(rp, ci, va) for rp, ci, va, _ in systems], class_policy="lossless")
     bp = B.pack([b for *_, b in systems])
     x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")   # captures the graph
-    reps = 3
     t_b = []
-    for _ in range(reps):
+    for _ in range(3):
         x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")
         t_b.append(ms)
     xs = B.unpack(x)
@@ -57,7 +61,6 @@ def run_extras(mf, torch, dev, log):
         worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(b)))
     conv = sum(1 for i in infos if i["status"] == 0)
     B.close()
-    # 512 sequential single-system launches of the same solver
     mats = [mf.TiledMatrix(rp, ci, va, class_policy="lossless") for rp, ci, va, _ in systems]
     bs = [torch.as_tensor(b, device=dev) for *_, b in systems]
     for M, bb in zip(mats[:8], bs[:8]):
@@ -76,16 +79,20 @@ def run_extras(mf, torch, dev, log):
         M.close()
     tb = float(np.median(t_b))
     alg = sum(b_ * i_["iterations"] for b_, i_ in zip(bpi, infos))
-    out["batched_512"] = {"systems": len(systems), "converged": conv, "worst_relres_true_host": worst,
-                          "launch_ms": tb, "solves_per_s": len(systems) / (tb * 1e-3),
-                          "iterations_total": int(sum(i_["iterations"] for i_ in infos)),
-                          "alg_GBs": alg / (tb * 1e-3) / 1e9,
-                          "sequential_kernel_ms": seq_kernel, "sequential_wall_ms": seq_wall,
-                          "sequential_solves_per_s": len(systems) / (seq_wall * 1e-3),
-                          "speedup_vs_sequential_wall": seq_wall / tb}
     log(f"[bench] batched 512: {tb:.2f} ms ({conv} converged, worst {worst:.2e}); "
         f"sequential {seq_wall:.1f} ms wall")
-    # differentiable solve, 64^3 variable-coefficient diffusion
+    return {"systems": len(systems), "converged": conv, "worst_relres_true_host": worst,
+            "launch_ms": tb, "solves_per_s": len(systems) / (tb * 1e-3),
+            "iterations_total": int(sum(i_["iterations"] for i_ in infos)),
+            "alg_GBs": alg / (tb * 1e-3) / 1e9,
+            "sequential_kernel_ms": seq_kernel, "sequential_wall_ms": seq_wall,
+            "sequential_solves_per_s": len(systems) / (seq_wall * 1e-3),
+            "speedup_vs_sequential_wall": seq_wall / tb}
+
+
+def run_autograd(mf, torch, dev, log):
+    """The differentiable solve: 64^3 variable-coefficient diffusion (k = exp(GRF), seed 4),
+    loss 0.5 ||x - x_t||^2, forward CG + adjoint CG through torch.autograd."""
     SparseOperator, sparse_solve = mf.autograd.SparseOperator, mf.autograd.sparse_solve
     rp, ci, va, _k = mf.workloads.varcoef_diffusion3d(64, seed=4)
     n = rp.size - 1
@@ -107,29 +114,66 @@ def run_extras(mf, torch, dev, log):
         vals.grad = None
         bt.grad = None
     fi, ai = op.last_info.get("forward", {}), op.last_info.get("adjoint", {})
-    out["autograd_64cubed"] = {"n": n, "nnz": int(ci.size), "forward_backward_ms": float(np.median(times)),
-                               "forward_iterations": fi.get("iterations"), "adjoint_iterations": ai.get("iterations"),
-                               "forward_kernel_ms": fi.get("time_ms"), "adjoint_kernel_ms": ai.get("time_ms"),
-                               "forward_alg_GBs": fi.get("gbs"), "adjoint_alg_GBs": ai.get("gbs")}
+    out = {"n": n, "nnz": int(ci.size), "forward_backward_ms": float(np.median(times)),
+           "forward_iterations": fi.get("iterations"), "adjoint_iterations": ai.get("iterations"),
+           "forward_kernel_ms": fi.get("time_ms"), "adjoint_kernel_ms": ai.get("time_ms"),
+           "forward_us_per_iter": fi.get("time_ms", 0) * 1e3 / max(1, fi.get("iterations", 1)),
+           "forward_alg_GBs": fi.get("gbs"), "adjoint_alg_GBs": ai.get("gbs")}
     log(f"[bench] autograd 64^3: forward+backward {np.median(times):.2f} ms "
-        f"(it {fi.get('iterations')} + {ai.get('iterations')})")
+        f"(it {fi.get('iterations')} + {ai.get('iterations')}, {out['forward_us_per_iter']:.2f} us/iteration)")
     op.close()
-    out["irregular_spd_1m"] = run_irregular(mf, torch, dev, log)
-    # the small CPU-comparable case: plain FP64 CG on 2D 5-point Poisson 64 x 64 (x* seed 0)
+    return out
+
+
+def run_poisson64(mf, torch, dev, log):
+    """Plain FP64 CG on 2D 5-point Poisson 64 x 64 (x* from seed 0), the CPU-comparable case."""
     rp, ci, va = mf.workloads.poisson2d(64)
     b_np, _ = mf.workloads.rhs_from_solution(rp, ci, va, 0)
     A = mf.TiledMatrix(rp, ci, va)
     bt = torch.as_tensor(b_np, device=dev)
     A.cg(bt, tol=1e-10, precision="fp64")
-    runs = [A.cg(bt, tol=1e-10, precision="fp64")[1] for _ in range(5)]
-    info = sorted(runs, key=lambda i: i["time_ms"])[2]
-    out["poisson2d_64_fp64"] = {"n": int(rp.size - 1), "nnz": int(ci.size), "iterations": info["iterations"],
-                                "kernel_ms": info["time_ms"], "relres_true": info["relres_true"],
-                                "alg_GBs": info["gbs"], "grid": info["grid"]}
+    runs = [A.cg(bt, tol=1e-10, precision="fp64") for _ in range(5)]
+    x, info = sorted(runs, key=lambda r: r[1]["time_ms"])[2]
+    r = b_np - mf.workloads.spmv(rp, ci, va, x.cpu().numpy())
+    out = {"n": int(rp.size - 1), "nnz": int(ci.size), "iterations": info["iterations"],
+           "kernel_ms": info["time_ms"], "us_per_iter": info["time_ms"] * 1e3 / max(1, info["iterations"]),
+           "relres_true_host": float(np.linalg.norm(r) / np.linalg.norm(b_np)),
+           "alg_GBs": info["gbs"], "grid": info["grid"]}
     log(f"[bench] 2D Poisson 64^2 FP64 CG: {info['time_ms']:.3f} ms, {info['iterations']} iterations, "
-        f"relres {info['relres_true']:.2e}")
+        f"relres {out['relres_true_host']:.2e}")
     A.close()
-    return out
+    return out, (rp, ci, va, b_np)
+
+
+def omp_env():
+    return {"OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "cpu": cpu_model(), "nproc": os.cpu_count()}
+
+
+def cpu_solve(mf, oracle, rp, ci, va, b_np, budget_s, full_iters, log, label):
+    """The reference's CPU path as intended (blockspmv_cpu.h:85-112 tiled traversal, host FP64
+    CG loop) on every host core (the oracle's OpenMP restatement).  Solves to 1e-10 when that
+    fits the budget, else times a bounded number of iterations and scales to full_iters."""
+    n = rp.size - 1
+    d = mf.tile_create(n, n, rp, ci, va)   # Tile_create arrays (bit-identical to the reference's)
+    d["_rowA"], d["_colA"] = n, n
+    probe = oracle.cg_tiled_omp(d, b_np, tol=0.0, maxit=5)
+    per_it = probe["seconds"] / 5
+    if per_it * full_iters <= budget_s:
+        res = oracle.cg_tiled_omp(d, b_np, tol=1e-10)
+        ms, k, how = res["seconds"] * 1e3, res["iterations"], "full solve to 1e-10"
+        scaled = ms
+    else:
+        k = max(5, int(budget_s / per_it))
+        res = oracle.cg_tiled_omp(d, b_np, tol=0.0, maxit=k)
+        ms = res["seconds"] * 1e3
+        scaled = ms / k * full_iters
+        how = f"{k} iterations timed ({ms / 1e3:.1f} s), scaled to the {full_iters} iterations of the solve"
+    log(f"[bench] cpu baseline {label}: {res['threads']} threads, {ms / k:.3f} ms/iteration -> {scaled:.1f} ms "
+        f"({how})")
+    return {"value": scaled, "unit": "ms", "cores": res["threads"], "kind": "port",
+            "sample": f"{label}: FP64 CG with the tiled blockspmv_cpu traversal (blockspmv_cpu.h:85-112, "
+                      f"OpenMP over tile rows, C): {how}", "ms_per_iter": ms / k, **omp_env()}
 
 
 def run_irregular(mf, torch, dev, log, iters=1000):
@@ -172,9 +216,10 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bicg", action="store_true")
-    ap.add_argument("--extras", action="store_true",
-                    help="also measure the 512-system batched solve and the 64^3 differentiable solve")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "cg", "bicg", "poisson64", "batched", "autograd", "irregular"])
+    ap.add_argument("--extras", action="store_true", help="also run the irregular SPD system at n = 10^6")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (reference kernel of known "
                          "traffic for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -199,25 +244,22 @@ def main():
     from __graft_entry__ import load_package
     mf = load_package()
     nx = args.grid
+    wl = args.workload
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    log(f"[bench] device {mf.device_name(0)}; building 3D Poisson {nx}^3")
-    rp, ci, va = mf.workloads.poisson3d(nx)
-    b_np, xs = mf.workloads.rhs_from_solution(rp, ci, va, seed=1)
-    n, nnz = rp.size - 1, ci.size
-    t0 = time.time()
-    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
-    t_conv = time.time() - t0
-    st = A.stats()
-    b = torch.as_tensor(b_np, device=dev)
-    log(f"[bench] n={n} nnz={nnz} tiles={st['tiles']} chunks={st['chunks']} "
-        f"classes={st['tiles_by_class']} conversion+upload {t_conv:.2f}s")
+    log(f"[bench] device {mf.device_name(0)}; workload {wl}")
+    if wl in ("poisson64", "batched", "autograd", "irregular"):
+        # one secondary workload alone (profiling runs)
+        fn = {"poisson64": lambda: run_poisson64(mf, torch, dev, log)[0],
+              "batched": lambda: run_batched(mf, torch, dev, log),
+              "autograd": lambda: run_autograd(mf, torch, dev, log),
+              "irregular": lambda: run_irregular(mf, torch, dev, log)}[wl]
+        print(json.dumps({"workload": wl, "result": fn()}), flush=True)
+        return
 
     def run(prec, steps, warmup, solver="cg", mat=None, rhs=None):
-        mat = mat or A
-        rhs = rhs if rhs is not None else b
         fn = mat.cg if solver == "cg" else mat.bicgstab
         for _ in range(warmup):
             fn(rhs, tol=1e-10, precision=prec)
@@ -231,11 +273,51 @@ def main():
         wall = (time.perf_counter() - tw) / steps * 1e3
         return x, infos, wall
 
-    x_m, inf_m, wall_m = run("mixed", args.steps, args.warmup)
+    def bicg_leg():
+        rp2, ci2, va2 = mf.workloads.convdiff3d(nx)
+        b2_np, _ = mf.workloads.rhs_from_solution(rp2, ci2, va2, seed=1)
+        A2 = mf.TiledMatrix(rp2, ci2, va2, class_policy="lossless")
+        st2 = A2.stats()
+        n2 = rp2.size - 1
+        b2 = torch.as_tensor(b2_np, device=dev)
+        xb_m, ib_m, _ = run("mixed", max(1, args.steps // 2), 1, "bicg", A2, b2)
+        xb_d, ib_d, _ = run("fp64", max(1, args.steps // 2), 1, "bicg", A2, b2)
+        tb_m = float(np.median([i["time_ms"] for i in ib_m]))
+        tb_d = float(np.median([i["time_ms"] for i in ib_d]))
+        rb = b2_np - mf.workloads.spmv(rp2, ci2, va2, xb_m.cpu().numpy())
+        itb = ib_m[-1]["iterations"]
+        res = {"mixed_ms": tb_m, "fp64_ms": tb_d, "speedup": tb_d / tb_m,
+               "iters_mixed": itb, "iters_fp64": ib_d[-1]["iterations"],
+               "relres_true_mixed": float(np.linalg.norm(rb) / np.linalg.norm(b2_np)),
+               "tiles_by_class": st2["tiles_by_class"],
+               "alg_GBs_mixed": (2 * st2["alg_bytes_mixed"] + 32.0 * n2) * itb / (tb_m * 1e-3) / 1e9,
+               "alg_GBs_fp64": (2 * st2["alg_bytes_fp64"] + 32.0 * n2) * ib_d[-1]["iterations"] / (tb_d * 1e-3) / 1e9}
+        log(f"[bench] BiCGSTAB convdiff: mixed {tb_m:.3f} ms ({itb} it) "
+            f"fp64 {tb_d:.3f} ms ({ib_d[-1]['iterations']} it)")
+        A2.close()
+        return res
+
+    if wl == "bicg":
+        print(json.dumps({"workload": wl, "result": bicg_leg()}), flush=True)
+        return
+
+    log(f"[bench] building 3D Poisson {nx}^3")
+    rp, ci, va = mf.workloads.poisson3d(nx)
+    b_np, xs = mf.workloads.rhs_from_solution(rp, ci, va, seed=1)
+    n, nnz = rp.size - 1, ci.size
+    t0 = time.time()
+    A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
+    t_conv = time.time() - t0
+    st = A.stats()
+    b = torch.as_tensor(b_np, device=dev)
+    log(f"[bench] n={n} nnz={nnz} tiles={st['tiles']} chunks={st['chunks']} "
+        f"classes={st['tiles_by_class']} conversion+upload {t_conv:.2f}s")
+
+    x_m, inf_m, wall_m = run("mixed", args.steps, args.warmup, mat=A, rhs=b)
     log(f"[bench] mixed CG: iters={inf_m[-1]['iterations']} repl={inf_m[-1]['replacements']} "
         f"kernel {np.median([i['time_ms'] for i in inf_m]):.3f} ms wall {wall_m:.3f} ms "
         f"relres_true={inf_m[-1]['relres_true']:.3e}")
-    x_d, inf_d, wall_d = run("fp64", args.steps, args.warmup)
+    x_d, inf_d, wall_d = run("fp64", args.steps, args.warmup, mat=A, rhs=b)
     log(f"[bench] fp64  CG: iters={inf_d[-1]['iterations']} "
         f"kernel {np.median([i['time_ms'] for i in inf_d]):.3f} ms wall {wall_d:.3f} ms "
         f"relres_true={inf_d[-1]['relres_true']:.3e}")
@@ -259,52 +341,28 @@ def main():
     gbs_d = bytes_fp64 / (t_fp64 * 1e-3) / 1e9
 
     extra = {}
-    if not args.no_bicg:
-        rp2, ci2, va2 = mf.workloads.convdiff3d(nx)
-        b2_np, _ = mf.workloads.rhs_from_solution(rp2, ci2, va2, seed=1)
-        A2 = mf.TiledMatrix(rp2, ci2, va2, class_policy="lossless")
-        st2 = A2.stats()
-        b2 = torch.as_tensor(b2_np, device=dev)
-        xb_m, ib_m, _ = run("mixed", max(1, args.steps // 2), 1, "bicg", A2, b2)
-        xb_d, ib_d, _ = run("fp64", max(1, args.steps // 2), 1, "bicg", A2, b2)
-        tb_m = float(np.median([i["time_ms"] for i in ib_m]))
-        tb_d = float(np.median([i["time_ms"] for i in ib_d]))
-        rb = b2_np - mf.workloads.spmv(rp2, ci2, va2, xb_m.cpu().numpy())
-        extra["bicgstab_convdiff"] = {
-            "mixed_ms": tb_m, "fp64_ms": tb_d, "speedup": tb_d / tb_m,
-            "iters_mixed": ib_m[-1]["iterations"], "iters_fp64": ib_d[-1]["iterations"],
-            "relres_true_mixed": float(np.linalg.norm(rb) / np.linalg.norm(b2_np)),
-            "tiles_by_class": st2["tiles_by_class"],
-            "alg_GBs_mixed": (2 * st2["alg_bytes_mixed"] + 32.0 * n) * ib_m[-1]["iterations"] / (tb_m * 1e-3) / 1e9,
-        }
-        log(f"[bench] BiCGSTAB convdiff: mixed {tb_m:.3f} ms ({ib_m[-1]['iterations']} it) "
-            f"fp64 {tb_d:.3f} ms ({ib_d[-1]['iterations']} it)")
-        A2.close()
-
-    if args.extras:
-        extra.update(run_extras(mf, torch, dev, log))
+    p64 = None
+    if wl == "all":
+        if not args.no_bicg:
+            extra["bicgstab_convdiff"] = bicg_leg()
+        extra["poisson2d_64_fp64"], p64 = run_poisson64(mf, torch, dev, log)
+        extra["batched_512"] = run_batched(mf, torch, dev, log)
+        extra["autograd_64cubed"] = run_autograd(mf, torch, dev, log)
+        if args.extras:
+            extra["irregular_spd_1m"] = run_irregular(mf, torch, dev, log)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and wl == "all":
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle  # the checker, timed as the CPU baseline
         oracle.build()
-        # bounded sample: K iterations of the oracle's FP64 CG on the same system,
-        # scaled to the FP64 iteration count of the GPU solve
-        k = 2
-        while True:
-            tc = time.perf_counter()
-            res = oracle.cg(rp, ci, va, b_np, tol=0.0, maxit=k)
-            dt = time.perf_counter() - tc
-            if dt > args.cpu_seconds / 4 or k >= it_d:
-                break
-            k = min(it_d, int(k * max(2.0, args.cpu_seconds / 2 / max(dt, 1e-3))))
-        per_it = dt / k
-        cpu = {"value": per_it * it_d * 1e3, "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"oracle_cg (textbook FP64 CG, C, 1 thread) on the same 128^3 system: {k} "
-                         f"iterations timed ({dt:.1f} s), scaled to the {it_d} iterations of the FP64 solve",
-               "cpu": cpu_model(), "nproc": os.cpu_count()}
-        log(f"[bench] cpu baseline: {per_it * 1e3:.2f} ms/iteration -> {cpu['value']:.0f} ms per solve")
+        cpu = cpu_solve(mf, oracle, rp, ci, va, b_np, args.cpu_seconds, it_d, log, f"3D Poisson {nx}^3")
+        cpu["gpu_speedup_mixed"] = cpu["value"] / t_mixed
+        if p64 is not None:
+            c1 = cpu_solve(mf, oracle, *p64, args.cpu_seconds, extra["poisson2d_64_fp64"]["iterations"], log,
+                           "2D Poisson 64^2")
+            c1["gpu_speedup_fp64"] = c1["value"] / extra["poisson2d_64_fp64"]["kernel_ms"]
+            extra["poisson2d_64_fp64"]["cpu_baseline"] = c1
 
     traffic = None
     tf = ROOT / "profiles" / "traffic_cg_poisson128_mixed.json"
@@ -333,6 +391,8 @@ def main():
         "fp64_ms": round(t_fp64, 4),
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 4),
         "iterations": {"mixed": it_m, "fp64": it_d, "replacements": repl},
+        "us_per_iter": {"mixed": round(t_mixed * 1e3 / (it_m + repl + 1), 3),
+                        "fp64": round(t_fp64 * 1e3 / (it_d + 1), 3)},
         "relres_true_host": {"mixed": rel_m, "fp64": rel_d},
         "alg_GBs": {"mixed": round(gbs_m, 1), "fp64": round(gbs_d, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs_m, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -7,6 +7,9 @@
 #include "mf_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -490,4 +493,96 @@ int oracle_static_class(const double *vals, int count)
         if (c < cls) cls = c;
     }
     return cls;
+}
+
+/* The reference's CPU path as intended (blockspmv_cpu.h:85-112 traversal, decoding the
+ * local column from Tile_csr_Col rather than the never-filled nibble array) inside a host
+ * FP64 CG loop: OpenMP over tile rows for the SpMV, OpenMP reductions for the dots.  The
+ * CPU baseline of the benchmark; returns like oracle_cg and reports the thread count. */
+static void tiled_spmv_omp(const oracle_tiles *t, int rowA, const double *x, double *y)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int bi = 0; bi < t->tilem; bi++) {
+        const int rl = rowlen_of(t, rowA, bi);
+        double acc[TILE];
+        for (int ri = 0; ri < TILE; ri++) acc[ri] = 0.0;
+        for (int k = t->tile_ptr[bi]; k < t->tile_ptr[bi + 1]; k++) {
+            const unsigned char *ptr = t->Blockcsr_Ptr + t->csrptr_offset[k];
+            const int nz = t->blknnz[k + 1] - t->blknnz[k];
+            const int off = t->csr_offset[k];
+            const double *xo = x + (size_t)t->tile_columnidx[k] * TILE;
+            for (int ri = 0; ri < rl; ri++) {
+                const int start = (rl == 1) ? 0 : ptr[ri];
+                const int stop = (ri == rl - 1) ? nz : ptr[ri + 1];
+                double s = 0.0;
+                for (int e = start; e < stop; e++) s += xo[t->Tile_csr_Col[off + e]] * t->Blockcsr_Val[off + e];
+                acc[ri] += s;
+            }
+        }
+        for (int ri = 0; ri < rl; ri++) y[bi * TILE + ri] = acc[ri];
+    }
+}
+
+static double dot_omp(int n, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+int oracle_cg_tiled_omp(const oracle_tiles *t, int rowA, const double *b, double *x, double tol, int maxit,
+                        int *iters, double *relres, int *threads)
+{
+    const int n = rowA;
+    const size_t pad = (size_t)(t->tilen > t->tilem ? t->tilen : t->tilem) * TILE + TILE;
+    double *r = malloc(sizeof(double) * pad), *p = calloc(pad, sizeof(double));
+    double *q = malloc(sizeof(double) * pad);
+#ifdef _OPENMP
+    *threads = omp_get_max_threads();
+#else
+    *threads = 1;
+#endif
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        x[i] = 0.0;
+        r[i] = b[i];
+        p[i] = b[i];
+    }
+    const double bnorm = sqrt(dot_omp(n, b, b));
+    double rr = dot_omp(n, r, r);
+    int it = 0, status = 1;
+    if (bnorm == 0.0) status = 0;
+    while (status == 1) {
+        if (sqrt(rr) <= tol * bnorm) {
+            status = 0;
+            break;
+        }
+        if (it >= maxit) break;
+        tiled_spmv_omp(t, rowA, p, q);
+        const double pq = dot_omp(n, p, q);
+        if (!(pq != 0.0) || !isfinite(pq)) {
+            status = 2;
+            break;
+        }
+        const double alpha = rr / pq;
+        double rr_new = 0.0;
+#pragma omp parallel for reduction(+ : rr_new) schedule(static)
+        for (int i = 0; i < n; i++) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            rr_new += r[i] * r[i];
+        }
+        const double beta = rr_new / rr;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+        rr = rr_new;
+        it++;
+    }
+    *iters = it;
+    *relres = bnorm > 0.0 ? sqrt(rr) / bnorm : 0.0;
+    free(r);
+    free(p);
+    free(q);
+    return status;
 }

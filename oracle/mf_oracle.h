@@ -92,6 +92,12 @@ int oracle_cg(int n, const int *rowptr, const int *colidx, const double *val,
               const double *b, double *x, double tol, int maxit, int *iters, double *relres,
               double *hist);
 
+/* The benchmark's CPU baseline: the same FP64 CG with the intended blockspmv_cpu tiled
+ * traversal (blockspmv_cpu.h:85-112, columns from Tile_csr_Col) and OpenMP over tile rows;
+ * *threads receives the OpenMP thread count.  Pinned against oracle_cg (tests). */
+int oracle_cg_tiled_omp(const oracle_tiles *t, int rowA, const double *b, double *x, double tol, int maxit,
+                        int *iters, double *relres, int *threads);
+
 /* Textbook FP64 BiCGSTAB (Mille-feuille-bicg.h:562-822; init main-bicg.cu:534-549). */
 int oracle_bicgstab(int n, const int *rowptr, const int *colidx, const double *val,
                     const double *b, double *x, double tol, int maxit, int *iters,

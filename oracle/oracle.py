@@ -83,6 +83,9 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                          ctypes.c_double, ctypes.c_int, _i32p, _f64p, _f64p]
             getattr(L, name).restype = ctypes.c_int
+        L.oracle_cg_tiled_omp.argtypes = [ctypes.POINTER(_Tiles), ctypes.c_int, _f64p, _f64p, ctypes.c_double,
+                                          ctypes.c_int, _i32p, _f64p, _i32p]
+        L.oracle_cg_tiled_omp.restype = ctypes.c_int
         L.oracle_dyn_code.argtypes = [ctypes.c_double]
         L.oracle_dyn_class.argtypes = [_f64p, _f64p, ctypes.c_int]
         L.oracle_static_class.argtypes = [_f64p, ctypes.c_int]
@@ -256,6 +259,26 @@ def cg(rowptr, colidx, val, b, tol=1e-10, maxit=100000, want_hist=False):
 def bicgstab(rowptr, colidx, val, b, tol=1e-10, maxit=100000, want_hist=False):
     """Textbook FP64 BiCGSTAB from x0 = 0 (Mille-feuille-bicg.h:562-822)."""
     return _solve("oracle_bicgstab", rowptr, colidx, val, b, tol, maxit, want_hist)
+
+
+def cg_tiled_omp(d, b, tol=1e-10, maxit=100000):
+    """The CPU baseline: FP64 CG over the tile arrays ``d`` (``tile_create``) with the intended
+    blockspmv_cpu traversal (blockspmv_cpu.h:85-112) and OpenMP on every host core
+    (OMP_NUM_THREADS).  Returns x, iterations, relres, threads, status and seconds."""
+    import time
+    L = lib()
+    t, keep = _to_struct(d)
+    n = int(d["_rowA"])
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    thr = ctypes.c_int(0)
+    t0 = time.perf_counter()
+    st = L.oracle_cg_tiled_omp(ctypes.byref(t), n, _p(b, _f64p), _p(x, _f64p), float(tol), int(maxit),
+                               ctypes.byref(it), ctypes.byref(rel), ctypes.byref(thr))
+    dt = time.perf_counter() - t0
+    return dict(status=st, x=x, iterations=it.value, relres=rel.value, threads=thr.value, seconds=dt)
 
 
 def dyn_code(delta):

@@ -341,3 +341,19 @@ def test_vectorised_mixed_emulation_matches_tile_emulation(orc):
     tile_of = np.repeat(np.arange(n), np.diff(rp)) // 16 * ((n + 15) // 16) + ci // 16
     for t in np.unique(tile_of)[:200]:
         assert np.unique(ecls[tile_of == t]).size == 1
+
+
+@pytest.mark.parametrize("nx", [40, 37])
+def test_tiled_openmp_cg_matches_oracle_cg(orc, nx):
+    """The CPU baseline (tiled traversal, OpenMP) runs the oracle's CG: same iteration count
+    within one (reduction order differs), same solution, on a ragged last tile row too."""
+    rp, ci, va = _poisson2d(nx)
+    n = rp.size - 1
+    b = _csr(rp, ci, va) @ np.random.default_rng(nx).uniform(-1, 1, n)
+    d = _tiles(orc, rp, ci, va)
+    ref = orc.cg(rp, ci, va, b, tol=1e-10)
+    got = orc.cg_tiled_omp(d, b, tol=1e-10)
+    assert got["status"] == 0 and got["threads"] >= 1
+    assert abs(got["iterations"] - ref["iterations"]) <= 1
+    assert np.allclose(got["x"], ref["x"], rtol=0, atol=1e-8)
+    assert orc.true_relres(rp, ci, va, got["x"], b) <= 1.5e-10
