@@ -310,9 +310,11 @@ mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const 
  * Results are those of mf_cg_solve with the same options (same kernel and grid). */
 mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                               mf_solve_info *info, void *stream);
-/* Autotuner of the persistent CG launch: times short fixed-iteration solves over
- * candidate workgroup counts and the four variants, caches the fastest for this matrix
- * and precision (used by later solves with grid = 0, variant = 0). */
+/* Autotuner of the persistent CG launch: times 40-iteration fixed-budget solves over
+ * candidate workgroup counts for each of the four variants, caches each variant's fastest
+ * grid and the fastest variant for this matrix and precision (used by later solves with
+ * grid = 0, and variant = 0).  d_x is not written (the trial iterates go to an internal
+ * vector; it may be NULL). */
 mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
                          int *best_variant, double *best_ms_per_iter, void *stream);
 /* Host-vector conveniences (copy in, solve, copy out, synchronise). */
@@ -345,8 +347,17 @@ mf_status mf_batch_row_offsets(const mf_batch *B, int64_t *offsets /* count+1 */
  * use_graph != 0 the launch is captured once into a hipGraph and replayed. */
 mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
                               int use_graph, mf_solve_info *per_system, double *time_ms, void *stream);
-/* Workgroups per system chosen by the autotuner (written after a tuned solve). */
+/* Bound on the rows one workgroup of a system holds (1024, 2048, 4096 or 8192): systems
+ * larger than that are split over 2, 4 or 8 workgroups with their own barrier group. */
 mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
+/* Autotuner of the batched launch (the reference's analogue is the partition search over
+ * nnz_list / block_per_warp, main-cg.cu:654-700): times `iterations`-iteration
+ * fixed-budget solves of every system (default 30) for each bound on the rows per
+ * workgroup (1024..8192, i.e. workgroups per system) and 1 or 2 chunk groups (16 tile
+ * chunks) per wave per SpMV batch, keeps the fastest in the batch for later solves and
+ * reports it.  The trial iterates go to an internal vector; d_b is only read. */
+mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int precision, int iterations, int *best_rows_per_wg,
+                            int *best_unroll, double *best_ms, void *stream);
 
 /* Library / device information. */
 int mf_device_count(void);

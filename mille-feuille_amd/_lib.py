@@ -152,6 +152,7 @@ _SIGS = {
     "mf_cg_solve_batched": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions), ctypes.c_int,
                                            ctypes.POINTER(SolveInfo), f64p, vp]),
     "mf_batch_set_groups": (ctypes.c_int, [vp, ctypes.c_int]),
+    "mf_batch_autotune": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, i32p, i32p, f64p, vp]),
     "mf_device_count": (ctypes.c_int, []),
     "mf_device_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
 }

@@ -395,6 +395,15 @@ class Batch:
     def set_rows_per_workgroup(self, r):
         check(L.lib().mf_batch_set_groups(self.h, int(r)), "mf_batch_set_groups")
 
+    def autotune(self, b_packed, precision="mixed", iterations=30):
+        """Pick rows per workgroup and chunk groups per wave for this batch (kept for later
+        solves); returns (rows_per_wg, unroll, ms of the fastest trial)."""
+        r, u, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(L.lib().mf_batch_autotune(self.h, ctypes.c_void_p(b_packed.data_ptr()), _PREC[precision], int(iterations),
+                                        ctypes.byref(r), ctypes.byref(u), ctypes.byref(t), _stream_handle()),
+              "mf_batch_autotune")
+        return r.value, u.value, t.value
+
     def pack(self, vectors):
         """Concatenate per-system vectors at the padded offsets (zeros in the padding)."""
         torch = _torch()

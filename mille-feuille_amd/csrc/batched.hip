@@ -157,7 +157,9 @@ struct mf_batch {
     std::vector<int> rows, tr0;        // per system
     std::vector<int64_t> row_off;      // count+1 (padded offsets)
     std::vector<int64_t> nnz;
-    int rows_per_wg = 8192;            // tuned bound on a member's rows
+    int rows_per_wg = 8192;            // bound on a member's rows (mf_batch_set_groups / autotuner)
+    int unroll = 0;                    // chunk groups (16 tile chunks each) per wave per SpMV batch;
+                                       // 0: the register-budget default (spmv_unroll)
     // device tables
     int *d_sys_tr = nullptr, *d_sys_rows = nullptr, *d_order = nullptr;
     int *d_pool_W = nullptr, *d_pool_wg0 = nullptr, *d_pool_sys0 = nullptr;
@@ -382,9 +384,25 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
 
 using BatchKernel = void (*)(mfk::BatchArgs);
 
-template <bool MIXED>
-static BatchKernel pick_batch(int rpt)
+// Kernel for rpt rows per thread and U chunk groups per wave per SpMV batch (U = 0: the
+// register-budget default; the autotuner tries 1 and 2).
+template <bool MIXED, int U>
+static BatchKernel pick_batch_u(int rpt)
 {
+    switch (rpt) {
+    case 1: return mfk::batched_cg_kernel<1, MIXED, U>;
+    case 2: return mfk::batched_cg_kernel<2, MIXED, U>;
+    case 4: return mfk::batched_cg_kernel<4, MIXED, U>;
+    case 8: return mfk::batched_cg_kernel<8, MIXED, U>;
+    default: return nullptr;
+    }
+}
+
+template <bool MIXED>
+static BatchKernel pick_batch(int rpt, int unroll)
+{
+    if (unroll == 1) return pick_batch_u<MIXED, 1>(rpt);
+    if (unroll == 2) return pick_batch_u<MIXED, 2>(rpt);
     switch (rpt) {
     case 1: return mfk::batched_cg_kernel<1, MIXED, mfk::spmv_unroll<1, 3>()>;
     case 2: return mfk::batched_cg_kernel<2, MIXED, mfk::spmv_unroll<2, 3>()>;
@@ -408,7 +426,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     if (opt) o = *opt;
     const bool mixed = o.precision == MF_PREC_MIXED && A->L.has_lowp;
     const int rpt = B->rows_per_wg / 1024;
-    BatchKernel k = mixed ? pick_batch<true>(rpt) : pick_batch<false>(rpt);
+    BatchKernel k = mixed ? pick_batch<true>(rpt, B->unroll) : pick_batch<false>(rpt, B->unroll);
     if (!k) return MF_ERR_INVALID;
     const size_t lds = (64 + 2 * (size_t)B->rows_per_wg) * sizeof(double);
     if (lds > (size_t)A->max_lds) return MF_ERR_TOO_LARGE;
@@ -545,4 +563,61 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
         }
     }
     return (mf_status)worst;
+}
+
+extern "C" mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int precision, int iterations,
+                                       int *best_rows_per_wg, int *best_unroll, double *best_ms, void *stream)
+{
+    if (!B || !d_b || !mf::aligned16(d_b)) return MF_ERR_INVALID;
+    mf_matrix *A = B->M;
+    if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
+    const int64_t n = B->row_off[B->count];
+    double *dx = nullptr;  // the trial solves' iterates (the caller's vectors are not written)
+    if (hipMalloc((void **)&dx, sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return MF_ERR_ALLOC;
+    mf_solve_options o;
+    mf_solve_options_default(&o);
+    o.precision = precision;
+    o.fixed_iterations = 1;  // every system runs the same budget: comparable work per candidate
+    o.max_iter = iterations > 0 ? iterations : 30;
+    const int rows0 = B->rows_per_wg, unroll0 = B->unroll;
+    double best = 1e30;
+    int br = rows0, bu = unroll0;
+    mf_status st = MF_SUCCESS;
+    for (int r : {1024, 2048, 4096, 8192}) {
+        for (int u : {1, 2}) {
+            if (r != B->rows_per_wg) B->free_plan();
+            B->rows_per_wg = r;
+            B->unroll = u;
+            double t = 1e30;
+            for (int rep = 0; rep < 2; rep++) {
+                double ms = 0;
+                st = mf_cg_solve_batched(B, d_b, dx, &o, 0, nullptr, &ms, stream);
+                if (st != MF_SUCCESS) break;
+                t = std::min(t, ms);
+            }
+            if (st != MF_SUCCESS) {  // e.g. a system beyond r rows per member on 8 members
+                st = MF_SUCCESS;
+                continue;
+            }
+            if (t < best) {
+                best = t;
+                br = r;
+                bu = u;
+            }
+        }
+    }
+    mf::free_dev(dx);
+    if (best >= 1e30) {
+        if (rows0 != B->rows_per_wg) B->free_plan();
+        B->rows_per_wg = rows0;
+        B->unroll = unroll0;
+        return MF_ERR_TOO_LARGE;
+    }
+    if (br != B->rows_per_wg) B->free_plan();
+    B->rows_per_wg = br;
+    B->unroll = bu;
+    if (best_rows_per_wg) *best_rows_per_wg = br;
+    if (best_unroll) *best_unroll = bu;
+    if (best_ms) *best_ms = best;
+    return MF_SUCCESS;
 }

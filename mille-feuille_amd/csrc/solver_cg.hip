@@ -379,14 +379,20 @@ extern "C" mf_status mf_cg_solve_host(mf_matrix *A, const double *b, double *x, 
 extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
                                     int *best_variant, double *best_ms_per_iter, void *stream)
 {
-    if (!A || !d_b || !d_x || !aligned16(d_b) || !aligned16(d_x)) return MF_ERR_INVALID;
+    (void)d_x;  // the trial iterates go to an internal vector: the caller's x is not written
+    if (!A || !d_b || !aligned16(d_b)) return MF_ERR_INVALID;
+    if (A->L.n_rows != A->L.n_cols) return MF_ERR_INVALID;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
     const bool mixed = precision == MF_PREC_MIXED && A->L.has_lowp;
+    double *dx = nullptr;
+    if (hipMalloc((void **)&dx, sizeof(double) * (size_t)A->L.n_rows + 16) != hipSuccess) return MF_ERR_ALLOC;
     const int cands[] = {8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 512};
     double best = 1e30;
     int bg = 0, bv = 2;
     for (int var = 1; var <= 4; var++) {
+        double bestv = 1e30;
+        int bgv = 0;
         for (int G : cands) {
             if (G > A->L.tilem || G > 2 * A->num_cus) continue;
             mf_solve_options o;
@@ -400,19 +406,27 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
             double t = 1e30;
             mf_status st = MF_SUCCESS;
             for (int rep = 0; rep < 2 && st == MF_SUCCESS; rep++) {
-                st = cg_launch(A, d_b, d_x, &o, &info, s);
+                st = cg_launch(A, d_b, dx, &o, &info, s);
                 if (st == MF_SUCCESS) t = std::min(t, info.time_ms / std::max(1, info.iterations));
             }
-            if (st != MF_SUCCESS) continue;  // e.g. rows per workgroup beyond the kernel's capacity
-            if (t < best) {
-                best = t;
-                bg = G;
-                bv = var;
+            // refused shapes are skipped: rows per workgroup beyond the kernel's capacity, or
+            // the pipelined variant beyond 2 rows per thread
+            if (st != MF_SUCCESS) continue;
+            if (t < bestv) {
+                bestv = t;
+                bgv = G;
             }
         }
+        if (!bgv) continue;
+        A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bgv;  // each variant keeps its own grid
+        if (bestv < best) {
+            best = bestv;
+            bg = bgv;
+            bv = var;
+        }
     }
+    free_dev(dx);
     if (bg == 0) return MF_ERR_TOO_LARGE;
-    for (int var = 1; var <= 4; var++) A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bg;
     A->tuned_variant[mixed ? 1 : 0] = bv;
     if (best_grid) *best_grid = bg;
     if (best_variant) *best_variant = bv;
