@@ -230,6 +230,9 @@ typedef struct mf_solve_options {
                               1 = tile chunks; 2 = sliced-ELL.  Sliced-ELL needs static classes
                               and every column within 32767 - 1024 (rows per thread - 1) of
                               its row; otherwise the tile chunks are used. */
+    int value_cache;       /* 0 = default: a variant-4 sliced-ELL CG whose workgroups' value
+                              streams fit their LDS next to x copies them there at launch
+                              (info->layout + 4); 1 = always read them from memory */
 } mf_solve_options;
 
 typedef struct mf_solve_info {
@@ -246,7 +249,8 @@ typedef struct mf_solve_info {
     double gbs;             /* bytes_per_iter * iterations / time */
     int layout;             /* SpMV layout used: 1 tile chunks, 2 sliced-ELL, 3 sliced-ELL with the
                                narrow unit profile; 4 and 5: 2 and 3 with each workgroup's rows
-                               sorted by length.  The recurrence variant is in `variant`. */
+                               sorted by length; 6 and 7: 2 and 3 with the value streams held in
+                               LDS.  The recurrence variant is in `variant`. */
     int variant;
     int precision;          /* arithmetic used: MF_PREC_MIXED, or MF_PREC_FP64 -- also for a
                                tile-mixed request when the matrix has no lower-precision tiles,

@@ -85,7 +85,7 @@ class SolveOptions(ctypes.Structure):
         ("fixed_iterations", ctypes.c_int), ("dynamic_precision", ctypes.c_int), ("replace_delta", ctypes.c_double),
         ("max_replacements", ctypes.c_int), ("grid", ctypes.c_int), ("block", ctypes.c_int), ("variant", ctypes.c_int),
         ("use_x0", ctypes.c_int), ("watchdog_ms", ctypes.c_double), ("transpose", ctypes.c_int),
-        ("layout", ctypes.c_int),
+        ("layout", ctypes.c_int), ("value_cache", ctypes.c_int),
     ]
 
 

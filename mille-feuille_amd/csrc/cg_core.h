@@ -106,7 +106,8 @@ __device__ __forceinline__ void spmv_op(const CgPart &a, Rows<RPT, BLK> &R, cons
                                         QVec<RPT, (SELL != 0)> &qv, double *scratch)
 {
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4), cg_pm<MIXED>()>(a.S, a.slot, R.base(), x64, x32, qv.reg);
+        spmv_sell<MIXED, RPT, BLK, sell_nar(SELL), cg_pm<MIXED>(), sell_vl(SELL, MIXED)>(a.S, a.slot, R.base(), x64,
+                                                                                          x32, qv.reg);
     } else {
         zero_q<RPT, BLK>(qv.lds, R);
         __syncthreads();
@@ -498,10 +499,11 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     unsigned pub = 0;       // flag epoch of the last p published without one
     // sliced-ELL: the first units of the next SpMV are loaded while the grid reduction,
     // the update and the flag wait run (the matrix does not change)
-    constexpr bool NAR = SELL == 2 || SELL == 4;
+    constexpr bool NAR = sell_nar(SELL), LDSV = sell_vl(SELL, MIXED);
     constexpr int PM = cg_pm<MIXED>();
     SellLead<sell_lead_stream<MIXED>(), NAR, PM> lead;
-    if constexpr (SELL != 0) sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
+    if constexpr (SELL != 0)
+        sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM, LDSV>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
     while (alive && res.status == ST_NOTCONV) {
         if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
             res.status = ST_TIMEOUT;
@@ -510,8 +512,9 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         p_visible = false;
         // q = A p
         if constexpr (SELL != 0) {
-            spmv_sell_lead<MIXED, RPT, BLK, NAR, PM>(a.S, a.slot, R.base(), a.p64, a.p32, q.reg, lead);
-            sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
+            spmv_sell_lead<MIXED, RPT, BLK, NAR, PM, LDSV>(a.S, a.slot, R.base(), a.p64, a.p32, q.reg, lead);
+            sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM, LDSV>(a.S.s[sell_lead_stream<MIXED>()], a.slot,
+                                                                         lead);
         } else {
             spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
         }
@@ -841,9 +844,14 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     R.row0 = tr0 * kTile;
     R.nrows = min(tr1 * kTile, a.A.n) - R.row0;
     // SELL: 0 tile chunks; 1/2 sliced-ELL, wide/narrow unit profile; 3/4 the same with each
-    // range's rows permuted (sorted by length)
-    R.perm = SELL >= 3 ? a.S.perm + (size_t)wg * RPT * BLK : nullptr;
+    // range's rows permuted (sorted by length); + kSellLds: value streams held in LDS
+    R.perm = sell_perm(SELL) ? a.S.perm + (size_t)wg * RPT * BLK : nullptr;
     R.refresh();
+    if constexpr (sell_lds(SELL)) {
+        static_assert(BLK == 1024, "the LDS copies are addressed by 16-wave workgroups");
+        sell_cache_values<RPT, sell_nar(SELL)>(a.S, wg);
+        __syncthreads();
+    }
     CgPart part;
     part.A = a.A;
     part.cr = a.A.wg_chunk + 4 * (size_t)wg;

@@ -522,6 +522,10 @@ static mf_status build_sell(const mf_matrix *A, const Partition &P, SellLayout &
         soff[st][slots] = (int)u;
         if (u >= INT32_MAX / 64) return MF_ERR_TOO_LARGE;
         SL.units[st] = u;
+        SL.max_wg_units[st] = 0;
+        for (int r = 0; r < R; r++)
+            SL.max_wg_units[st] = std::max(SL.max_wg_units[st], soff[st][(size_t)(r + 1) * NW * rpt] -
+                                                                    soff[st][(size_t)r * NW * rpt]);
         const int E = mfk::sell_e_host(st, SL.narrow);
         idx[st].assign((size_t)u * 64 * E, 0);
         val[st].assign((size_t)u * 64 * E * mfk::sell_vbytes_host(st), 0);
@@ -713,6 +717,7 @@ mf_status ensure_sell(mf_matrix *A, const Partition *Pc, int rpt, bool mixed, hi
         out->s[k].val = SL.val[k];
         out->s[k].lmask = SL.lmask[k];
         out->s[k].xmax = A->L.n_rows - 1;
+        out->s[k].lds = -1;  // the launcher decides which streams a kernel holds in LDS
     }
     out->perm = SL.perm;
     *narrow = SL.narrow;

@@ -40,7 +40,29 @@ struct SellStream {
     const void *val;   // [unit][64][E] values at the stream's width
     const int *lmask;  // per (range, wave): 0 when every lane holds the same column words, else 63
     int xmax;          // last column (n - 1)
+    // kernels that hold values on chip: byte offset in the workgroup's dynamic LDS of its
+    // copy of this stream's value words (its units, in order), or -1 (read from memory)
+    int lds;
 };
+
+// SELL code of the persistent kernels: 0 tile chunks; bits 0-2: 1 wide unit profile,
+// 2 narrow, 3 / 4 the same with each range's rows sorted by length; bit 3 (kSellLdsClass):
+// the class streams' values (tile-mixed SpMV) are copied into LDS at launch and every
+// SpMV reads them from there; bit 4 (kSellLds64): the same for the binary64 stream.  The
+// choice is compile-time per stream: a runtime choice inside the kernel kept both code
+// paths' registers live and spilled (69 VGPRs).
+constexpr int kSellLdsClass = 8, kSellLds64 = 16;
+__host__ __device__ constexpr bool sell_nar(int SELL) { return (SELL & 7) == 2 || (SELL & 7) == 4; }
+__host__ __device__ constexpr bool sell_perm(int SELL) { return (SELL & 7) >= 3; }
+__host__ __device__ constexpr bool sell_lds(int SELL) { return (SELL & (kSellLdsClass | kSellLds64)) != 0; }
+// whether the SpMV over the class streams (MIXED) or the binary64 stream reads LDS copies
+__host__ __device__ constexpr bool sell_vl(int SELL, bool MIXED)
+{
+    return (SELL & (MIXED ? kSellLdsClass : kSellLds64)) != 0;
+}
+
+// The kernels' dynamic LDS, seen as 32-bit words (the cached value streams live in it).
+extern __shared__ unsigned sell_smem_u32[];
 
 struct DevSell {
     SellStream s[kSellStreams];
@@ -169,12 +191,52 @@ struct SellUnit {
 };
 
 // ilane: the lane whose column words this lane reads (lane & the wave's lmask)
-template <int S, bool NAR, int PM>
-__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR, PM> &U, int u, int lane, int ilane)
+// Words i*NW .. i*NW+NW-1 of the LDS region starting at byte `off` (one ds_read).
+template <int NW>
+__device__ __forceinline__ Words<NW> lds_words(int off, int i)
+{
+    Words<NW> r;
+    const int w = (off >> 2) + i * NW;
+    if constexpr (NW == 1) {
+        r.w[0] = sell_smem_u32[w];
+    } else if constexpr (NW == 2) {
+        const u32x2 t = *reinterpret_cast<const u32x2 *>(&sell_smem_u32[w]);
+        r.w[0] = t.x;
+        r.w[1] = t.y;
+    } else {
+        const u32x4 t = *reinterpret_cast<const u32x4 *>(&sell_smem_u32[w]);
+        r.w[0] = t.x;
+        r.w[1] = t.y;
+        r.w[2] = t.z;
+        r.w[3] = t.w;
+    }
+    return r;
+}
+
+// Column and value words of unit u (VL: the values from the workgroup's LDS copy, whose
+// first unit is uw0).
+template <int S, bool NAR, int PM, bool VL = false>
+__device__ __forceinline__ void sell_load(const SellStream &st, SellUnit<S, NAR, PM> &U, int u, int lane, int ilane,
+                                          int uw0 = 0)
 {
     U.d = load_words<SellUnit<S, NAR, PM>::NWI>(st.idx, (size_t)u * 64 + ilane);
-    U.v = load_words<SellUnit<S, NAR, PM>::NWV>(st.val, (size_t)u * 64 + lane);
+    if constexpr (VL) U.v = lds_words<SellUnit<S, NAR, PM>::NWV>(st.lds, (u - uw0) * 64 + lane);
+    else U.v = load_words<SellUnit<S, NAR, PM>::NWV>(st.val, (size_t)u * 64 + lane);
 }
+
+// First unit of the calling wave's workgroup in a stream (its 16 waves' slots are
+// consecutive: slot = workgroup * 16 + wave).
+template <int RPT>
+__device__ __forceinline__ int sell_wg_unit0(const SellStream &st, int slot)
+{
+    return __builtin_amdgcn_readfirstlane(st.soff[(size_t)(slot & ~15) * RPT]);
+}
+
+// Copies the value words of the workgroup's units of every stream with st.lds >= 0 into
+// its LDS region (all threads, 16-byte pieces; the regions are 16-byte aligned).  The
+// caller synchronises the workgroup before the first SpMV.
+template <int RPT, bool NAR>
+__device__ __forceinline__ void sell_cache_values(const struct DevSell &D, int wg);
 
 template <int S, bool NAR, int PM>
 __device__ __forceinline__ void sell_gather(SellUnit<S, NAR, PM> &U, int row, const SellX &X)
@@ -224,8 +286,8 @@ struct SellLead {
     SellUnit<S, NAR, PM> u0, u1;
 };
 
-template <int S, int RPT, bool NAR, int PM>
-__device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, SellLead<S, NAR, PM> &L)
+template <int S, int RPT, bool NAR, int PM, bool VL = false>
+__device__ __forceinline__ void sell_prefetch_impl(const SellStream &st, int slot, SellLead<S, NAR, PM> &L)
 {
     const int lane = threadIdx.x & 63;
     const int *sb = st.soff + (size_t)slot * RPT;
@@ -233,8 +295,16 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
     const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
     if (u0 > last) return;  // the wave has no unit in this stream (the SpMV skips it too)
     const int ilane = lane & __builtin_amdgcn_readfirstlane(st.lmask[slot]);
-    sell_load<S, NAR, PM>(st, L.u0, u0, lane, ilane);
-    sell_load<S, NAR, PM>(st, L.u1, min(u0 + 1, last), lane, ilane);
+    const int uw0 = VL ? sell_wg_unit0<RPT>(st, slot) : 0;
+    sell_load<S, NAR, PM, VL>(st, L.u0, u0, lane, ilane, uw0);
+    sell_load<S, NAR, PM, VL>(st, L.u1, min(u0 + 1, last), lane, ilane, uw0);
+}
+
+// VL: the stream's values are read from the workgroup's LDS copy
+template <int S, int RPT, bool NAR, int PM, bool VL = false>
+__device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, SellLead<S, NAR, PM> &L)
+{
+    sell_prefetch_impl<S, RPT, NAR, PM, VL>(st, slot, L);
 }
 
 // q[k] += (A x)[row0 + BLK k + 64 v + lane] over stream S's entries.  Column words are
@@ -254,10 +324,10 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
 // q's order.  Unit bounds are wave-uniform (scalar loads).
 // LEAD: the first two units were loaded by sell_prefetch into *lead (whose registers the
 // pipeline then reuses: prefetch again before the next use).
-template <int S, int RPT, int BLK, bool NAR, int PM = kPmTiles, bool LEAD = false>
-__device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
-                                            const float *__restrict__ x32, double (&q)[RPT],
-                                            SellLead<S, NAR, PM> *lead = nullptr)
+template <int S, int RPT, int BLK, bool NAR, int PM = kPmTiles, bool LEAD = false, bool VL = false>
+__device__ __forceinline__ void sell_stream_impl(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
+                                                 const float *__restrict__ x32, double (&q)[RPT],
+                                                 SellLead<S, NAR, PM> *lead = nullptr)
 {
     const int lane = threadIdx.x & 63;
     const int *sb = st.soff + (size_t)slot * RPT;
@@ -265,6 +335,7 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
     const int last = __builtin_amdgcn_readfirstlane(sb[RPT]) - 1;
     if (u0 > last) return;
     const int ilane = lane & __builtin_amdgcn_readfirstlane(st.lmask[slot]);
+    const int uw0 = VL ? sell_wg_unit0<RPT>(st, slot) : 0;
     const SellX X = sell_x(x64, x32, st.xmax + 1);
     // ends of the slices not yet closed, in scalar registers (e[0]: the current one)
     int e[RPT];
@@ -289,15 +360,15 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
     double acc = 0.0;
     int i = u0;
     if constexpr (!LEAD) {
-        sell_load<S, NAR, PM>(st, S0, i, lane, ilane);
-        sell_load<S, NAR, PM>(st, S1, min(i + 1, last), lane, ilane);
+        sell_load<S, NAR, PM, VL>(st, S0, i, lane, ilane, uw0);
+        sell_load<S, NAR, PM, VL>(st, S1, min(i + 1, last), lane, ilane, uw0);
     }
     sell_gather<S, NAR, PM>(S0, rowb, X);
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
     // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
     auto step = [&](UT &a, UT &nb1, UT &nb2) {
-        sell_load<S, NAR, PM>(st, nb2, min(i + 2, last), lane, ilane);
+        sell_load<S, NAR, PM, VL>(st, nb2, min(i + 2, last), lane, ilane, uw0);
         __builtin_amdgcn_sched_barrier(0);
         sell_gather<S, NAR, PM>(nb1, rowb, X);
         __builtin_amdgcn_sched_barrier(0);
@@ -326,6 +397,35 @@ __device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int 
         __builtin_amdgcn_sched_barrier(0);
     } while (i <= last);
     while (rot < RPT) close_slice();  // trailing empty slices
+}
+
+// One stream's contribution; VL: values from the workgroup's LDS copy.
+template <int S, int RPT, int BLK, bool NAR, int PM = kPmTiles, bool LEAD = false, bool VL = false>
+__device__ __forceinline__ void sell_stream(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
+                                            const float *__restrict__ x32, double (&q)[RPT],
+                                            SellLead<S, NAR, PM> *lead = nullptr)
+{
+    sell_stream_impl<S, RPT, BLK, NAR, PM, LEAD, VL>(st, slot, rowb, x64, x32, q, lead);
+}
+
+template <int RPT, bool NAR>
+__device__ __forceinline__ void sell_cache_values(const DevSell &D, int wg)
+{
+    const int E[kSellStreams] = {sell_e<0, NAR>(), sell_e<1, NAR>(), sell_e<2, NAR>(), sell_e<3, NAR>(),
+                                 sell_e<4, NAR>()};
+    const int VB[kSellStreams] = {sell_vbytes<0>(), sell_vbytes<1>(), sell_vbytes<2>(), sell_vbytes<3>(),
+                                  sell_vbytes<4>()};
+#pragma unroll
+    for (int s = 0; s < kSellStreams; s++) {
+        const SellStream &st = D.s[s];
+        if (st.lds < 0) continue;
+        const int u0 = st.soff[(size_t)wg * 16 * RPT], u1 = st.soff[(size_t)(wg + 1) * 16 * RPT];
+        const int bytes_per_unit = 64 * E[s] * VB[s];  // a multiple of 64
+        const long long n16 = (long long)(u1 - u0) * bytes_per_unit / 16;
+        const u32x4 *src = reinterpret_cast<const u32x4 *>((const char *)st.val + (size_t)u0 * bytes_per_unit);
+        u32x4 *dst = reinterpret_cast<u32x4 *>(&sell_smem_u32[st.lds >> 2]);
+        for (long long i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = gptr<u32x4>(src)[i];
+    }
 }
 
 // The SpMV result q = A v for a thread's rows: in registers (sliced-ELL SpMV, SELL = true)
@@ -365,7 +465,9 @@ struct RowVec {
 // mixed SpMV sums first) or binary64.
 template <bool MIXED> constexpr int sell_lead_stream() { return MIXED ? 3 : kSellAll64; }
 
-template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles>
+// VL: the streams this SpMV reads (class streams when MIXED, else the binary64 one) have LDS
+// copies in the workgroup (sell_vl of the kernel's SELL code)
+template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles, bool LDSV = false>
 __device__ __forceinline__ void spmv_sell_lead(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                                const float *__restrict__ x32, double (&q)[RPT],
                                                SellLead<sell_lead_stream<MIXED>(), NAR, PM> &lead)
@@ -373,28 +475,28 @@ __device__ __forceinline__ void spmv_sell_lead(const DevSell &D, int slot, int r
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {
-        sell_stream<3, RPT, BLK, NAR, PM, true>(D.s[3], slot, rowb, x64, x32, q, &lead);
-        sell_stream<0, RPT, BLK, NAR, PM>(D.s[0], slot, rowb, x64, x32, q);
-        sell_stream<1, RPT, BLK, NAR, PM>(D.s[1], slot, rowb, x64, x32, q);
-        sell_stream<2, RPT, BLK, NAR, PM>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK, NAR, PM, true, LDSV>(D.s[3], slot, rowb, x64, x32, q, &lead);
+        sell_stream<0, RPT, BLK, NAR, PM, false, LDSV>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR, PM, false, LDSV>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR, PM, false, LDSV>(D.s[2], slot, rowb, x64, x32, q);
     } else {
-        sell_stream<4, RPT, BLK, NAR, PM, true>(D.s[4], slot, rowb, x64, x32, q, &lead);
+        sell_stream<4, RPT, BLK, NAR, PM, true, LDSV>(D.s[4], slot, rowb, x64, x32, q, &lead);
     }
 }
 
-template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles>
+template <bool MIXED, int RPT, int BLK, bool NAR = false, int PM = kPmTiles, bool LDSV = false>
 __device__ __forceinline__ void spmv_sell(const DevSell &D, int slot, int rowb, const double *__restrict__ x64,
                                           const float *__restrict__ x32, double (&q)[RPT])
 {
 #pragma unroll
     for (int k = 0; k < RPT; k++) q[k] = 0.0;
     if constexpr (MIXED) {
-        sell_stream<0, RPT, BLK, NAR, PM>(D.s[0], slot, rowb, x64, x32, q);
-        sell_stream<1, RPT, BLK, NAR, PM>(D.s[1], slot, rowb, x64, x32, q);
-        sell_stream<2, RPT, BLK, NAR, PM>(D.s[2], slot, rowb, x64, x32, q);
-        sell_stream<3, RPT, BLK, NAR, PM>(D.s[3], slot, rowb, x64, x32, q);
+        sell_stream<0, RPT, BLK, NAR, PM, false, LDSV>(D.s[0], slot, rowb, x64, x32, q);
+        sell_stream<1, RPT, BLK, NAR, PM, false, LDSV>(D.s[1], slot, rowb, x64, x32, q);
+        sell_stream<2, RPT, BLK, NAR, PM, false, LDSV>(D.s[2], slot, rowb, x64, x32, q);
+        sell_stream<3, RPT, BLK, NAR, PM, false, LDSV>(D.s[3], slot, rowb, x64, x32, q);
     } else {
-        sell_stream<4, RPT, BLK, NAR, PM>(D.s[4], slot, rowb, x64, x32, q);
+        sell_stream<4, RPT, BLK, NAR, PM, false, LDSV>(D.s[4], slot, rowb, x64, x32, q);
     }
 }
 

@@ -82,7 +82,7 @@ __device__ __forceinline__ void bicg_spmv(const BicgArgs &a, const long long *cr
 {
     constexpr int PM = MIXED ? kPmWide64 : kPmTiles;
     if constexpr (SELL != 0) {
-        spmv_sell<MIXED, RPT, BLK, (SELL == 2 || SELL == 4), PM>(a.S, slot, R.base(), x64, x32, q.reg);
+        spmv_sell<MIXED, RPT, BLK, sell_nar(SELL), PM>(a.S, slot, R.base(), x64, x32, q.reg);
     } else {
         zero_q<RPT, BLK>(q.lds, R);
         __syncthreads();

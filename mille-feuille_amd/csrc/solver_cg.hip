@@ -118,6 +118,30 @@ static CgKernel pick_cg_perm(int rpt)
 
 static bool cg_perm_supported(int rpt, int var) { return (var == 2 || var == 4) && (rpt == 4 || rpt == 8); }
 
+// Variant-4 sliced-ELL kernels whose value streams live in LDS (identity row order).
+static CgKernel cg_sell_lds_kernel_for(int rpt, bool mixed, bool narrow)
+{
+    if (mixed) return narrow ? pick_cg<4, true, false, false, 2 | mfk::kSellLdsClass>(rpt)
+                             : pick_cg<4, true, false, false, 1 | mfk::kSellLdsClass>(rpt);
+    return pick_cg<4, false, false, false, 1 | mfk::kSellLds64>(rpt);
+}
+
+// LDS bytes of one range's copy of the value streams a solve reads (class streams 0..3 for
+// tile-mixed arithmetic, the binary64 stream otherwise), each region 16-byte aligned.
+static size_t sell_lds_bytes(const SellLayout &SL, bool mixed_arith, size_t offsets[mfk::kSellStreams], size_t base)
+{
+    size_t off = (base + 15) / 16 * 16;
+    for (int k = 0; k < mfk::kSellStreams; k++) {
+        offsets[k] = (size_t)-1;
+        const bool used = mixed_arith ? k < mfk::kSellAll64 : k == mfk::kSellAll64;
+        if (!used || SL.max_wg_units[k] == 0) continue;
+        offsets[k] = off;
+        off += (size_t)SL.max_wg_units[k] * 64 * mfk::sell_e_host(k, SL.narrow) * mfk::sell_vbytes_host(k);
+        off = (off + 15) / 16 * 16;
+    }
+    return off;
+}
+
 static CgKernel cg_sell_kernel_for(int rpt, int var, bool mixed, bool narrow, bool perm)
 {
     if (perm) {
@@ -264,6 +288,23 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
         if (bc >= 0.95 * b64) mixed_arith = false;
     }
     CgKernel k = sell ? cg_sell_kernel_for(rpt, var, mixed_arith, narrow, dsell.perm != nullptr) : nullptr;
+    // On-chip value streams (the reference's "shared queue", Mille-feuille-cg.h:586,
+    // main-cg-mixed.cu:484): when one range's copy of the streams the solve reads fits the
+    // workgroup's LDS next to x, the variant-4 kernel copies them in at launch and every SpMV
+    // reads them from LDS instead of L2 / HBM.
+    bool on_chip = false;
+    if (sell && k && var == 4 && !dsell.perm && o->value_cache == 0) {
+        size_t offs[mfk::kSellStreams];
+        const size_t need = sell_lds_bytes(*P->sell, mixed_arith, offs, lds_x);
+        CgKernel kl = cg_sell_lds_kernel_for(rpt, mixed_arith, narrow);
+        // (4 KB left for the kernel's static LDS: reduction partials and flags)
+        if (kl && need + 4096 <= (size_t)A->max_lds) {
+            for (int s2 = 0; s2 < mfk::kSellStreams; s2++) dsell.s[s2].lds = offs[s2] == (size_t)-1 ? -1 : (int)offs[s2];
+            k = kl;
+            lds = need;
+            on_chip = true;
+        }
+    }
     if (sell && !k && dsell.perm) {
         // the partition's streams were built permuted for another variant: tile chunks
         sell = false;
@@ -323,7 +364,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
         info->grid = G;
         info->block = B;
         info->rows_per_thread = rpt;
-        info->layout = sell ? (mixed_arith && narrow ? 3 : 2) + (dsell.perm ? 2 : 0) : 1;
+        info->layout = sell ? (mixed_arith && narrow ? 3 : 2) + (dsell.perm ? 2 : 0) + (on_chip ? 4 : 0) : 1;
         info->variant = var;
         info->precision = mixed_arith ? MF_PREC_MIXED : MF_PREC_FP64;
         const double M = mixed_arith ? A->L.alg_bytes_mixed : A->L.alg_bytes_f64;

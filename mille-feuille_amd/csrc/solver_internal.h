@@ -32,6 +32,7 @@ struct SellLayout {
     void *val[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int *lmask[mfk::kSellStreams] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per wave slot
     int64_t shared_units[mfk::kSellStreams] = {0, 0, 0, 0, 0};  // units of shared-column waves
+    int max_wg_units[mfk::kSellStreams] = {0, 0, 0, 0, 0};  // most units one range holds, per stream
     double bytes = 0;               // device bytes of the built streams
 };
 

@@ -75,8 +75,9 @@ def test_bicgstab_mixed_follows_fp64(mf, orc, torch, layout, nx):
     recurrence on the same values, summed in another order (class streams).  On these
     symmetric systems BiCGSTAB's count is that sensitive to summation order alone: the
     oracle takes 515 iterations at 256^2 and 508 with every row's entries reversed (SciPy's
-    bicgstab 548); at 512^2 1,099, 887 and 1,018.  So the counts are held to 20 % of the
-    FP64 solve's (the 3D convection-diffusion system holds 2 %), with both at 1e-10."""
+    bicgstab 548); at 512^2 1,099, 887 and 1,018 (24 % apart).  So the counts are held to
+    30 % of the FP64 solve's (the 3D convection-diffusion system holds 2 %), with both at
+    1e-10."""
     rp, ci, va = mf.workloads.poisson2d(nx)
     b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
     A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
@@ -86,7 +87,7 @@ def test_bicgstab_mixed_follows_fp64(mf, orc, torch, layout, nx):
     print(f"  fp64 {i64['iterations']} mixed {im['iterations']} (repl {im['replacements']})", flush=True)
     assert i64["status"] == 0 and im["status"] == 0, (i64, im)
     assert im["precision"] == 1
-    assert abs(im["iterations"] - i64["iterations"]) <= 0.2 * i64["iterations"]
+    assert abs(im["iterations"] - i64["iterations"]) <= 0.3 * i64["iterations"]
     assert orc.true_relres(rp, ci, va, x64.cpu().numpy(), b) <= 1.2e-10
     assert orc.true_relres(rp, ci, va, xm.cpu().numpy(), b) <= 1.2e-10
     A.close()
