@@ -315,10 +315,12 @@ mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const 
 mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double *d_x, const mf_solve_options *opt,
                               mf_solve_info *info, void *stream);
 /* Autotuner of the persistent CG launch: times 40-iteration fixed-budget solves over
- * candidate workgroup counts for each of the four variants, caches each variant's fastest
- * grid and the fastest variant for this matrix and precision (used by later solves with
- * grid = 0, and variant = 0).  d_x is not written (the trial iterates go to an internal
- * vector; it may be NULL). */
+ * candidate workgroup counts for each of the four variants and caches each variant's
+ * fastest grid; then runs each variant on its grid to a 1e-10 relative residual and caches
+ * the variant of the fastest converged solve for this matrix and precision (used by later
+ * solves with grid = 0 and variant = 0).  *best_ms_per_iter is that solve's time per
+ * iteration.  d_x is not written (the trial iterates go to an internal vector; it may be
+ * NULL). */
 mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x, int precision, int *best_grid,
                          int *best_variant, double *best_ms_per_iter, void *stream);
 /* Host-vector conveniences (copy in, solve, copy out, synchronise). */
@@ -362,6 +364,14 @@ mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
  * reports it.  The trial iterates go to an internal vector; d_b is only read. */
 mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int precision, int iterations, int *best_rows_per_wg,
                             int *best_unroll, double *best_ms, void *stream);
+
+/* Diagnostics (no counterpart in the reference).  Per-phase timers of the last
+ * variant-4 single-kernel CG launch: n_wg x 8 counters per logical workgroup, in
+ * s_memrealtime ticks (100 MHz): neighbour wait, SpMV, local dots, grid reduction,
+ * update + p publish, residual replacements, iteration count, and the ticks from entering
+ * a grid reduction to publishing the workgroup's record.  Only libmf_phase.so (the same
+ * sources built with -DMF_PHASE_TIMERS) fills them; libmf.so returns MF_ERR_UNSUPPORTED. */
+mf_status mf_phase_read(unsigned long long *out, int n_wg);
 
 /* Library / device information. */
 int mf_device_count(void);

@@ -153,9 +153,20 @@ _SIGS = {
                                            ctypes.POINTER(SolveInfo), f64p, vp]),
     "mf_batch_set_groups": (ctypes.c_int, [vp, ctypes.c_int]),
     "mf_batch_autotune": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, i32p, i32p, f64p, vp]),
+    "mf_phase_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
     "mf_device_count": (ctypes.c_int, []),
     "mf_device_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
 }
+
+
+def use_library(name):
+    """Load ``name`` (a library of this directory built from the same sources, e.g.
+    ``libmf_phase.so`` with per-phase timers) instead of libmf.so; only before the first
+    call into the library."""
+    global LIB_PATH
+    if _LIB is not None:
+        raise RuntimeError(f"{LIB_PATH.name} is already loaded")
+    LIB_PATH = PKG_DIR / name
 
 
 def lib():

@@ -461,6 +461,8 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     res.status = ST_NOTCONV;
     res.rr = res.bb = 0.0;
     res.rt = -1.0;
+    PhaseClock pc;  // per-phase timers (empty unless built with them)
+    pc.begin(sy.wg);
     double r[RPT], p[RPT];
     if (a.use_x0) spmv_op<false, U, false, SELL>(a, R, a.x, nullptr, q, scratch);
     double part[2] = {0.0, 0.0};
@@ -505,10 +507,12 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
     if constexpr (SELL != 0)
         sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM, LDSV>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
     while (alive && res.status == ST_NOTCONV) {
+        pc.start();
         if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
             res.status = ST_TIMEOUT;
             break;
         }
+        pc.mark(0);
         p_visible = false;
         // q = A p
         if constexpr (SELL != 0) {
@@ -518,6 +522,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         } else {
             spmv_op<MIXED, U, false, SELL>(a, R, a.p64, a.p32, q, scratch);
         }
+        pc.mark(1);
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         R.refresh();
 #pragma unroll
@@ -529,10 +534,12 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
             d[2] += qk * qk;
             d[3] += r[k] * r[k];
         }
+        pc.sync_mark(2);
         if (!sy.template sum<4>(d, false, scratch)) {
             res.status = ST_TIMEOUT;
             break;
         }
+        pc.mark(3);
         rr = d[3];
         if (!a.fixed) {
             bool conv = rr <= a.tol2 * bb;
@@ -546,6 +553,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
                 conv = rr <= a.tol2 * bb;
                 if (!conv) {
                     p_visible = true;  // the corrected p went out with a grid barrier
+                    pc.mark(5);
                     continue;
                 }
             }
@@ -577,7 +585,10 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         publish_p<RPT, BLK>(a, R, p);
         nb_signal(a.nb, sy.wg, ++pub);
         res.it++;
+        pc.mark(4);
+        pc.iter();
     }
+    pc.flush(sy.wg);
     // write x, then the binary64 true residual ||b - A x||^2 (its first grid barrier also
     // covers a p published without one)
     if (res.status != ST_TIMEOUT) {

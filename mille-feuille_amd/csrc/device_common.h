@@ -55,6 +55,65 @@ struct GridSync {
 
 __device__ __forceinline__ void wait_own_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
 
+// Phase clock of the persistent CG iteration.  Compiled in only with -DMF_PHASE_TIMERS
+// (the separate libmf_phase.so); otherwise every member is empty and the kernels are
+// unchanged.  Thread 0 of each workgroup accumulates s_memrealtime ticks (a constant
+// 100 MHz clock) per phase into g_phase[wg * kPhaseSlots + phase]; slot kPhaseIters counts
+// the iterations and slot kPhaseRecord the ticks from entering a grid reduction to this
+// member's record being published.  The timers add one __syncthreads before each grid
+// reduction, so the phases add up to slightly more than the uninstrumented iteration.
+constexpr int kPhaseSlots = 8;
+constexpr int kPhaseMaxWg = 1024;
+constexpr int kPhaseIters = 6;
+constexpr int kPhaseRecord = 7;
+#ifdef MF_PHASE_TIMERS
+static __device__ unsigned long long g_phase[kPhaseMaxWg * kPhaseSlots];
+struct PhaseClock {
+    unsigned long long t = 0, acc[kPhaseIters] = {0, 0, 0, 0, 0, 0};
+    unsigned long long its = 0;
+    __device__ __forceinline__ void begin(int wg)
+    {
+        if (threadIdx.x == 0 && wg < kPhaseMaxWg) g_phase[(size_t)wg * kPhaseSlots + kPhaseRecord] = 0;
+    }
+    __device__ __forceinline__ void start() { t = __builtin_amdgcn_s_memrealtime(); }
+    __device__ __forceinline__ void mark(int k)
+    {
+        const unsigned long long n = __builtin_amdgcn_s_memrealtime();
+        acc[k] += n - t;
+        t = n;
+    }
+    __device__ __forceinline__ void sync_mark(int k)
+    {
+        __syncthreads();
+        mark(k);
+    }
+    __device__ __forceinline__ void iter() { its++; }
+    __device__ __forceinline__ void flush(int wg)
+    {
+        if (threadIdx.x != 0 || wg >= kPhaseMaxWg) return;
+        for (int k = 0; k < kPhaseIters; k++) g_phase[(size_t)wg * kPhaseSlots + k] = acc[k];
+        g_phase[(size_t)wg * kPhaseSlots + kPhaseIters] = its;
+    }
+};
+__device__ __forceinline__ unsigned long long phase_now() { return __builtin_amdgcn_s_memrealtime(); }
+__device__ __forceinline__ void phase_record(int wg, unsigned long long t0)
+{
+    if (threadIdx.x == 0 && wg < kPhaseMaxWg)
+        g_phase[(size_t)wg * kPhaseSlots + kPhaseRecord] += __builtin_amdgcn_s_memrealtime() - t0;
+}
+#else
+struct PhaseClock {
+    __device__ __forceinline__ void begin(int) {}
+    __device__ __forceinline__ void start() {}
+    __device__ __forceinline__ void mark(int) {}
+    __device__ __forceinline__ void sync_mark(int) {}
+    __device__ __forceinline__ void iter() {}
+    __device__ __forceinline__ void flush(int) {}
+};
+__device__ __forceinline__ unsigned long long phase_now() { return 0; }
+__device__ __forceinline__ void phase_record(int, unsigned long long) {}
+#endif
+
 // Logical workgroup id: consecutive logical ids on the same XCD (dispatch is
 // round-robin over the 8 XCDs), so each XCD owns a contiguous band of rows and
 // the p gathers of neighbouring rows stay inside one L2.
@@ -101,23 +160,100 @@ __device__ __forceinline__ void store_wt(T *p, T v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// v from lane (lane ^ M) of the wave, through the LDS crossbar (ds_bpermute).
+template <int M>
+__device__ __forceinline__ double xor_move(double v)
+{
+    const int addr = ((threadIdx.x & 63) ^ M) << 2;
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Lanes per group of the split reduction of K values (K rounded up to 1, 2 or 4).
+template <int K>
+__device__ __host__ constexpr int split_width() { return K <= 1 ? 1 : (K <= 2 ? 2 : 4); }
+
+// Split reduction of K per-lane values: the KP lanes of each aligned group trade values
+// (DPP quad permutations) so that lane l keeps value k = l % KP summed over its group --
+// one exchange per halving instead of a full wave reduction per value.  Values k >= K
+// are zero.
+template <int K>
+__device__ __forceinline__ double wave_split(const double (&v)[K])
+{
+    constexpr int KP = split_width<K>();
+    const int lane = threadIdx.x & 63;
+    if constexpr (KP == 1) {
+        return v[0];
+    } else if constexpr (KP == 2) {
+        const bool odd = lane & 1;
+        return (odd ? v[1] : v[0]) + dpp_move<0xB1>(odd ? v[0] : v[1]);  // quad_perm [1,0,3,2]
+    } else {
+        const double v0 = v[0], v1 = v[1], v2 = K > 2 ? v[2] : 0.0, v3 = K > 3 ? v[3] : 0.0;
+        const bool odd = lane & 1, high = lane & 2;
+        // lane ^ 1: even lanes keep values 0 and 2, odd lanes 1 and 3
+        const double a = (odd ? v1 : v0) + dpp_move<0xB1>(odd ? v0 : v1);
+        const double b = (odd ? v3 : v2) + dpp_move<0xB1>(odd ? v2 : v3);
+        // lane ^ 2 (quad_perm [2,3,0,1]): lanes with bit 1 clear keep a, the others b
+        return (high ? b : a) + dpp_move<0x4E>(high ? a : b);
+    }
+}
+
+// Wave totals of the split values: lane k (< K) ends with the sum over the wave of value
+// k.  Within each 16-lane row the lanes holding the same value are combined by DPP
+// rotations, then the four rows by two crossbar exchanges; every lane position sums in
+// the same order, so the result depends only on the inputs.
+template <int KP>
+__device__ __forceinline__ double wave_join(double w)
+{
+    w += dpp_move<0x128>(w);                         // row_ror:8
+    w += dpp_move<0x124>(w);                         // row_ror:4
+    if constexpr (KP <= 2) w += dpp_move<0x122>(w);  // row_ror:2
+    if constexpr (KP == 1) w += dpp_move<0x121>(w);  // row_ror:1
+    w += xor_move<16>(w);
+    w += xor_move<32>(w);
+    return w;
+}
+
+// Per-wave partials of K values into part[k * 16 + wave] (16 waves at most).
+template <int K>
+__device__ __forceinline__ void wave_partials(const double (&v)[K], double *part)
+{
+    const double w = wave_join<split_width<K>()>(wave_split<K>(v));
+    const int lane = threadIdx.x & 63;
+    if (lane < K) part[lane * 16 + (threadIdx.x >> 6)] = w;
+}
+
+// Called by all 64 lanes of one wave after a barrier: lane 16 k returns the block total of
+// value k (the nwv wave partials of part[], combined in a fixed order by DPP rotations).
+template <int K>
+__device__ __forceinline__ double block_combine(const double *part, int nwv)
+{
+    const int lane = threadIdx.x & 63, k = lane >> 4, i = lane & 15;
+    double w = (k < K && i < nwv) ? part[k * 16 + i] : 0.0;
+    w += dpp_move<0x128>(w);  // row_ror:8
+    w += dpp_move<0x124>(w);  // row_ror:4
+    w += dpp_move<0x122>(w);  // row_ror:2
+    w += dpp_move<0x121>(w);  // row_ror:1
+    return w;
+}
+
 // Deterministic sum of K per-thread values over the block; result in every thread.
+// scratch: at least 16 K doubles.
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K], double *scratch)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) scratch[k * 16 + wv] = s;
+    __shared__ double s_tot[kMaxRed];
+    const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
+    wave_partials<K>(v, scratch);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double t = block_combine<K>(scratch, nwv);
+        if ((lane & 15) == 0 && (lane >> 4) < K) s_tot[lane >> 4] = t;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = 0.0;
-        for (int i = 0; i < nwv; i++) s += scratch[k * 16 + i];
-        v[k] = s;
-    }
+    for (int k = 0; k < K; k++) v[k] = s_tot[k];
     __syncthreads();
 }
 
@@ -200,24 +336,23 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
     __shared__ double s_part[kMaxRed * 16];  // per-wave partials of this member
     __shared__ double s_bc[kMaxRed];         // the group result
     __shared__ int s_fail;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
+    // lane 16 k of wave 0 carries value k through the block combines below
+    const int kl = lane >> 4;
+    const bool klead = threadIdx.x < 64 && (lane & 15) == 0 && kl < K;
+    const unsigned long long ph0 = phase_now();
     if (visibility) wait_own_stores();
     if (threadIdx.x == 0) s_fail = 0;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const double w = wave_sum(v[k]);
-        if (lane == 0) s_part[k * 16 + wv] = w;
-    }
+    wave_partials<K>(v, s_part);
     __syncthreads();
     epoch++;
     const int G = gs.G;
     if (G == 1) {
         // the workgroup is the grid: its barrier is the grid barrier (each wave waited for
         // its write-through stores before it), and an L1 invalidate makes them visible
-        if (threadIdx.x < K) {
-            double m = 0.0;
-            for (int i = 0; i < nwv; i++) m += s_part[threadIdx.x * 16 + i];
-            s_bc[threadIdx.x] = m;
+        if (threadIdx.x < 64) {
+            const double m = block_combine<K>(s_part, nwv);
+            if (klead) s_bc[kl] = m;
         }
         if (visibility && gs.acq && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
@@ -230,11 +365,11 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
     // wave 0 combines the wave partials in a fixed order and publishes this member's
     // record; with visibility every wave has waited for its write-through stores before
     // the barrier above, so they are complete
-    if (threadIdx.x < K) {
-        double m = 0.0;
-        for (int i = 0; i < nwv; i++) m += s_part[threadIdx.x * 16 + i];
-        put_record(rec_at(gs, epoch, threadIdx.x, wg), m, epoch);
+    if (threadIdx.x < 64) {
+        const double m = block_combine<K>(s_part, nwv);
+        if (klead) put_record(rec_at(gs, epoch, kl, wg), m, epoch);
     }
+    phase_record(wg, ph0);
     // keep the compiler from moving the record stores behind this thread's polling
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (wg == 0) {
@@ -253,18 +388,15 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
             for (int kk = 0; kk < K; kk++)
                 if (kk == k) acc[kk] += d;
         }
-        // fixed order: per thread, then per wave (DPP), then over the waves
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const double w = wave_sum(acc[k]);
-            if (lane == 0) scratch[k * 16 + wv] = w;
-        }
+        // fixed order: per thread, then per wave (split reduction), then over the waves
+        wave_partials<K>(acc, scratch);
         __syncthreads();
-        if (threadIdx.x < K) {
-            double t = 0.0;
-            for (int i = 0; i < nwv; i++) t += scratch[threadIdx.x * 16 + i];
-            put_record(rec_at(gs, epoch, threadIdx.x, G), t, s_fail ? (epoch | kAbortTag) : epoch);
-            s_bc[threadIdx.x] = t;
+        if (threadIdx.x < 64) {
+            const double t = block_combine<K>(scratch, nwv);
+            if (klead) {
+                put_record(rec_at(gs, epoch, kl, G), t, s_fail ? (epoch | kAbortTag) : epoch);
+                s_bc[kl] = t;
+            }
         }
         // after the broadcast (off the other members' critical path): drop stale L2 lines
         if (visibility && gs.acq && threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");

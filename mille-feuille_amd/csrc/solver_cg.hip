@@ -430,7 +430,9 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
     if (hipMalloc((void **)&dx, sizeof(double) * (size_t)A->L.n_rows + 16) != hipSuccess) return MF_ERR_ALLOC;
     const int cands[] = {8, 16, 32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 512};
     double best = 1e30;
-    int bg = 0, bv = 2;
+    int bg = 0, bv = 2, best_it = 0;
+    int grid_of[5] = {0, 0, 0, 0, 0};
+    double per_it[5] = {1e30, 1e30, 1e30, 1e30, 1e30};
     for (int var = 1; var <= 4; var++) {
         double bestv = 1e30;
         int bgv = 0;
@@ -460,14 +462,49 @@ extern "C" mf_status mf_cg_autotune(mf_matrix *A, const double *d_b, double *d_x
         }
         if (!bgv) continue;
         A->tuned_grid[((long long)var << 8) | (mixed ? 1 : 0)] = bgv;  // each variant keeps its own grid
-        if (bestv < best) {
-            best = bestv;
-            bg = bgv;
+        grid_of[var] = bgv;
+        per_it[var] = bestv;
+    }
+    // The variants differ in iteration count and stability (the pipelined recurrence in
+    // binary32 products can stagnate on ill-conditioned systems), so each variant's best
+    // shape is then run to convergence (tolerance 1e-10) and the fastest converged solve
+    // wins; a later variant that would need more than twice the best iteration count so
+    // far is cut off there.
+    int cap = 100000;
+    const int order[] = {4, 2, 1, 3};
+    for (int var : order) {
+        if (!grid_of[var]) continue;
+        mf_solve_options o;
+        mf_solve_options_default(&o);
+        o.precision = precision;
+        o.variant = var;
+        o.grid = grid_of[var];
+        o.tol = 1e-10;
+        o.max_iter = cap;
+        mf_solve_info info;
+        if (cg_launch(A, d_b, dx, &o, &info, s) != MF_SUCCESS || info.status != MF_SUCCESS) continue;
+        if (info.time_ms < best) {
+            best = info.time_ms;
+            bg = grid_of[var];
             bv = var;
+            best_it = info.iterations;
         }
+        cap = std::min(cap, 2 * std::max(1, info.iterations));
     }
     free_dev(dx);
-    if (bg == 0) return MF_ERR_TOO_LARGE;
+    if (bg == 0) {
+        // nothing converged within the caps: the fastest iteration
+        for (int var = 1; var <= 4; var++)
+            if (grid_of[var] && per_it[var] < best) {
+                best = per_it[var];
+                bg = grid_of[var];
+                bv = var;
+                best_it = 0;
+            }
+        if (bg == 0) return MF_ERR_TOO_LARGE;
+    } else {
+        best /= std::max(1, best_it);
+    }
     A->tuned_variant[mixed ? 1 : 0] = bv;
     if (best_grid) *best_grid = bg;
     if (best_variant) *best_variant = bv;
@@ -548,4 +585,27 @@ extern "C" mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double
     w.dvis = (uint8_t *)(base + kNaDvis);
     w.cap_g = kNaMaxGrid;
     return cg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream, &w);
+}
+
+// Per-phase timers of the persistent CG (variant 4), filled only by the library built with
+// -DMF_PHASE_TIMERS (libmf_phase.so).  Copies n_wg x 8 counters of the last launch on the
+// current device: per logical workgroup, s_memrealtime ticks (100 MHz) of [0] the wait for
+// the neighbours' p, [1] the SpMV, [2] the local dot products up to a block barrier, [3] the
+// grid reduction, [4] the update and p publish, [5] residual replacements, then [6] the
+// iteration count and [7] the ticks from entering a grid reduction to publishing this
+// workgroup's record (summed over every reduction of the launch).
+extern "C" mf_status mf_phase_read(unsigned long long *out, int n_wg)
+{
+#ifdef MF_PHASE_TIMERS
+    if (!out || n_wg <= 0 || n_wg > mfk::kPhaseMaxWg) return MF_ERR_INVALID;
+    if (hipDeviceSynchronize() != hipSuccess) return MF_ERR_HIP;
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(mfk::g_phase), sizeof(unsigned long long) * mfk::kPhaseSlots * n_wg) ==
+                   hipSuccess
+               ? MF_SUCCESS
+               : MF_ERR_HIP;
+#else
+    (void)out;
+    (void)n_wg;
+    return MF_ERR_UNSUPPORTED;
+#endif
 }
