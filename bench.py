@@ -50,6 +50,14 @@ def run_batched(mf, torch, dev, log):
     B = mf.Batch([(rp, ci, va) for rp, ci, va, _ in systems], class_policy="lossless")
     bp = B.pack([b for *_, b in systems])
     x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")   # captures the graph
+    t_d = []
+    for _ in range(3):
+        x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")
+        t_d.append(ms)
+    # the batched autotuner: rows per workgroup (workgroups per system) and chunk groups
+    # per wave, tried on 30-iteration launches and kept for the batch
+    rows, unroll, _ = B.autotune(bp, precision="mixed", iterations=30)
+    x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")   # captures the tuned graph
     t_b = []
     for _ in range(3):
         x, infos, ms = B.cg(bp, use_graph=True, tol=1e-10, precision="mixed")
@@ -79,10 +87,12 @@ def run_batched(mf, torch, dev, log):
         M.close()
     tb = float(np.median(t_b))
     alg = sum(b_ * i_["iterations"] for b_, i_ in zip(bpi, infos))
-    log(f"[bench] batched 512: {tb:.2f} ms ({conv} converged, worst {worst:.2e}); "
-        f"sequential {seq_wall:.1f} ms wall")
+    log(f"[bench] batched 512: {tb:.2f} ms tuned (rows/wg {rows}, unroll {unroll}), {np.median(t_d):.2f} ms "
+        f"default ({conv} converged, worst {worst:.2e}); sequential {seq_wall:.1f} ms wall")
     return {"systems": len(systems), "converged": conv, "worst_relres_true_host": worst,
-            "launch_ms": tb, "solves_per_s": len(systems) / (tb * 1e-3),
+            "launch_ms": tb, "default_launch_ms": float(np.median(t_d)),
+            "tuned": {"rows_per_wg": rows, "chunk_groups_per_wave": unroll},
+            "solves_per_s": len(systems) / (tb * 1e-3),
             "iterations_total": int(sum(i_["iterations"] for i_ in infos)),
             "alg_GBs": alg / (tb * 1e-3) / 1e9,
             "sequential_kernel_ms": seq_kernel, "sequential_wall_ms": seq_wall,
@@ -138,9 +148,18 @@ def run_poisson64(mf, torch, dev, log):
     out = {"n": int(rp.size - 1), "nnz": int(ci.size), "iterations": info["iterations"],
            "kernel_ms": info["time_ms"], "us_per_iter": info["time_ms"] * 1e3 / max(1, info["iterations"]),
            "relres_true_host": float(np.linalg.norm(r) / np.linalg.norm(b_np)),
-           "alg_GBs": info["gbs"], "grid": info["grid"]}
+           "alg_GBs": info["gbs"], "grid": info["grid"], "layout": info["layout"], "variant": info["variant"]}
+    # the single-system autotuner (workgroups and CG variant, verified to converge)
+    A.autotune(bt, precision="fp64")
+    runs = [A.cg(bt, tol=1e-10, precision="fp64") for _ in range(5)]
+    xt_, it_ = sorted(runs, key=lambda r: r[1]["time_ms"])[2]
+    rt = b_np - mf.workloads.spmv(rp, ci, va, xt_.cpu().numpy())
+    out["tuned"] = {"kernel_ms": it_["time_ms"], "iterations": it_["iterations"], "grid": it_["grid"],
+                    "variant": it_["variant"], "layout": it_["layout"],
+                    "relres_true_host": float(np.linalg.norm(rt) / np.linalg.norm(b_np))}
     log(f"[bench] 2D Poisson 64^2 FP64 CG: {info['time_ms']:.3f} ms, {info['iterations']} iterations, "
-        f"relres {out['relres_true_host']:.2e}")
+        f"relres {out['relres_true_host']:.2e}; tuned (grid {it_['grid']}, variant {it_['variant']}) "
+        f"{it_['time_ms']:.3f} ms")
     A.close()
     return out, (rp, ci, va, b_np)
 

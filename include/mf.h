@@ -223,10 +223,15 @@ typedef struct mf_solve_options {
                               barrier (four per iteration). */
     int use_x0;            /* x holds an initial guess (else x0 = 0) */
     double watchdog_ms;    /* grid-barrier wait bound (0 = default 10000 ms) */
-    int transpose;         /* solve with A^T (same tile stream read transposed) */
+    int transpose;         /* solve A^T x = b: the solver runs on A^T as a tiled matrix of its
+                              own (same class policy), built on the matrix's first transposed
+                              call from the host copy of its CSR and kept until
+                              mf_matrix_update_values (the reference's CSR transpose,
+                              utils.h:172) */
     int layout;            /* single-kernel CG / BiCGSTAB SpMV layout: 0 = default (row-per-lane
-                              sliced-ELL streams for tile-mixed CG and for BiCGSTAB, tile
-                              chunks for all-FP64 CG);
+                              sliced-ELL streams for tile-mixed CG, for BiCGSTAB, and for
+                              all-FP64 CG whose binary64 stream fits in LDS; tile chunks for
+                              the other all-FP64 CG);
                               1 = tile chunks; 2 = sliced-ELL.  Sliced-ELL needs static classes
                               and every column within 32767 - 1024 (rows per thread - 1) of
                               its row; otherwise the tile chunks are used. */
@@ -284,8 +289,9 @@ mf_status mf_pattern_outer(mf_matrix *A, const double *d_u, const double *d_v, d
 mf_status mf_dyn_classify(const double *d_p_old, const double *d_p_new, int n, int mode, double rms,
                           unsigned char *d_out, void *stream);
 
-/* y = A x (or A^T x) on device pointers.  precision MF_PREC_MIXED applies the tile
- * classes (binary32 products, FP64 sums); MF_PREC_FP64 reads the FP64 stream. */
+/* y = A x (or, with transpose, A^T x: the SpMV of the transposed tiled matrix described
+ * under mf_solve_options.transpose) on device pointers.  precision MF_PREC_MIXED applies
+ * the tile classes (binary32 products, FP64 sums); MF_PREC_FP64 reads the FP64 stream. */
 mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int precision, int transpose,
                   void *stream);
 
@@ -305,7 +311,9 @@ mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *d_x, const 
  *   - n_rows <= 2,097,152 and at most 512 workgroups (else MF_ERR_TOO_LARGE);
  *   - the matrix's launch plan for these options (its partition for the grid and, for
  *     sliced-ELL solves, the SpMV streams) must exist: an earlier mf_cg_solve with the
- *     same options on this handle builds it; otherwise MF_ERR_UNSUPPORTED;
+ *     same options on this handle builds it; otherwise MF_ERR_UNSUPPORTED (with transpose
+ *     set, the transposed matrix and its plan must exist: an earlier transposed
+ *     mf_cg_solve builds them);
  *   - variant 3 (pipelined) is refused (MF_ERR_UNSUPPORTED) and info->time_ms is 0
  *     (no timing events);
  *   - every call on one device shares the array: calls that overlap on a device race

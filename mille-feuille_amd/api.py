@@ -202,7 +202,7 @@ def _dev_vec(v, n, device):
 
 def solve_options(tol=1e-10, max_iter=100000, precision="fp64", fixed_iterations=False, dynamic_precision=0,
                   replace_delta=1e-3, max_replacements=64, grid=0, variant=0, use_x0=False, watchdog_ms=0.0,
-                  layout="auto", value_cache=True):
+                  layout="auto", value_cache=True, transpose=False):
     o = L.SolveOptions()
     L.lib().mf_solve_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -218,6 +218,7 @@ def solve_options(tol=1e-10, max_iter=100000, precision="fp64", fixed_iterations
     o.watchdog_ms = float(watchdog_ms)
     o.layout = {"auto": 0, "chunks": 1, "sell": 2}[layout] if isinstance(layout, str) else int(layout)
     o.value_cache = 0 if value_cache else 1
+    o.transpose = int(bool(transpose))
     return o
 
 

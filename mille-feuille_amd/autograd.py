@@ -3,13 +3,18 @@
 ``SparseSolve.apply(vals, b, op)`` returns x.  Forward runs the single-kernel CG
 (SPD) or BiCGSTAB (general) of libmf.so and stores only x.  Backward solves the
 adjoint system A^T lambda = g with the same kernel -- reusing A itself for SPD CG,
-a transposed tiled matrix for BiCGSTAB -- and returns
+the library's transposed tiled matrix (solve option ``transpose``) for BiCGSTAB --
+and returns
 
     grad_b    = lambda
     grad_vals = -lambda_i * x_j   on the sparsity pattern (CSR order), written by a
                                   HIP kernel of the library (mf_pattern_outer).
 
-Both solves run to the operator's tolerance (default 1e-10 relative residual).
+Both solves run to the operator's tolerance (default 1e-10 relative residual).  The
+operator keeps a device copy of the values it was last built with: a call with the same
+values compares them on the device and leaves the tiled matrix as it is; changed values
+go through the library's value update (tile classes are a host decision, as in the
+reference's Tile_create).
 """
 from __future__ import annotations
 
@@ -36,38 +41,33 @@ class SparseOperator:
         self.class_policy = class_policy
         v = np.ascontiguousarray(vals, np.float64)
         self.A = TiledMatrix(self.rowptr, self.colidx, v, class_policy=class_policy, device=device)
-        self._vals = v.copy()
+        self._dvals = None  # device copy of the values the matrix holds (set on first use)
+        self._init_vals = v
         self.version = 0  # bumped whenever the matrix's values change
-        self.AT = None
-        if not self.spd:
-            # transposed pattern and the permutation from its CSR order to ours
-            rows = np.repeat(np.arange(self.n), np.diff(self.rowptr))
-            order = np.lexsort((rows, self.colidx))
-            self.t_perm = order
-            self.t_rowptr = np.zeros(self.n + 1, np.int64)
-            np.add.at(self.t_rowptr, self.colidx + 1, 1)
-            self.t_rowptr = np.cumsum(self.t_rowptr).astype(np.int32)
-            self.t_colidx = rows[order].astype(np.int32)
-            self.AT = TiledMatrix(self.t_rowptr, self.t_colidx, v[order], class_policy=class_policy, device=device)
         self.last_info = {}
 
     def set_values(self, vals):
-        v = vals.detach().to("cpu", dtype=__import__("torch").float64).numpy() if hasattr(vals, "detach") \
-            else np.asarray(vals, np.float64)
-        if v.shape != (self.nnz,):
+        """Make the matrix hold ``vals`` (device tensor, or host array) in CSR order."""
+        import torch
+
+        dev = torch.device("cuda", self.device)
+        t = vals.detach() if hasattr(vals, "detach") else torch.as_tensor(np.asarray(vals, np.float64))
+        t = t.to(device=dev, dtype=torch.float64)
+        if t.shape != (self.nnz,):
             raise ValueError("values do not match the pattern")
-        if np.array_equal(v, self._vals):
+        if self._dvals is None:
+            self._dvals = torch.as_tensor(self._init_vals, device=dev)
+            self._init_vals = None
+        if torch.equal(t, self._dvals):  # on the device: no host copy of unchanged values
             return
-        self._vals = v.copy()
+        self._dvals = t.clone()
         self.version += 1
-        self.A.update_values(v)
-        if self.AT is not None:
-            self.AT.update_values(v[self.t_perm])
+        self.A.update_values(t.cpu().numpy())
 
     def solve(self, b, transpose=False):
-        M = self.AT if (transpose and not self.spd) else self.A
-        kw = dict(tol=self.tol, max_iter=self.max_iter, precision=self.precision)
-        x, info = (M.cg(b, **kw) if self.spd else M.bicgstab(b, **kw))
+        kw = dict(tol=self.tol, max_iter=self.max_iter, precision=self.precision,
+                  transpose=bool(transpose) and not self.spd)
+        x, info = (self.A.cg(b, **kw) if self.spd else self.A.bicgstab(b, **kw))
         if info["status"] != 0:
             raise L.MfError(info["status"], "adjoint solve" if transpose else "solve")
         self.last_info["adjoint" if transpose else "forward"] = info
@@ -78,8 +78,6 @@ class SparseOperator:
 
     def close(self):
         self.A.close()
-        if self.AT is not None:
-            self.AT.close()
 
 
 def _make_function():

@@ -57,42 +57,56 @@ __device__ __forceinline__ void wait_own_stores() { __builtin_amdgcn_s_waitcnt(0
 
 // Phase clock of the persistent CG iteration.  Compiled in only with -DMF_PHASE_TIMERS
 // (the separate libmf_phase.so); otherwise every member is empty and the kernels are
-// unchanged.  Thread 0 of each workgroup accumulates s_memrealtime ticks (a constant
-// 100 MHz clock) per phase into g_phase[wg * kPhaseSlots + phase]; slot kPhaseIters counts
-// the iterations and slot kPhaseRecord the ticks from entering a grid reduction to this
-// member's record being published.  The timers add one __syncthreads before each grid
-// reduction, so the phases add up to slightly more than the uninstrumented iteration.
+// unchanged.  Thread 0 of each workgroup reads s_memrealtime (a constant 100 MHz clock) at
+// the phase boundaries and accumulates the ticks per phase in LDS (no registers live
+// across the iteration, so the instrumented kernel keeps the register allocation of the
+// plain one), then copies them to g_phase[wg * kPhaseSlots + phase]; slot kPhaseIters
+// counts the iterations and slot kPhaseRecord the ticks from entering a grid reduction to
+// this member's record being published.  The timers add one __syncthreads before each
+// grid reduction.
 constexpr int kPhaseSlots = 8;
 constexpr int kPhaseMaxWg = 1024;
 constexpr int kPhaseIters = 6;
 constexpr int kPhaseRecord = 7;
 #ifdef MF_PHASE_TIMERS
 static __device__ unsigned long long g_phase[kPhaseMaxWg * kPhaseSlots];
+__device__ __forceinline__ unsigned long long *phase_lds()
+{
+    __shared__ unsigned long long s_phase[kPhaseSlots];  // [0, 6) phases, 6 iterations, 7 last mark
+    return s_phase;
+}
 struct PhaseClock {
-    unsigned long long t = 0, acc[kPhaseIters] = {0, 0, 0, 0, 0, 0};
-    unsigned long long its = 0;
     __device__ __forceinline__ void begin(int wg)
     {
-        if (threadIdx.x == 0 && wg < kPhaseMaxWg) g_phase[(size_t)wg * kPhaseSlots + kPhaseRecord] = 0;
+        if (threadIdx.x != 0) return;
+        for (int k = 0; k < kPhaseSlots; k++) phase_lds()[k] = 0;
+        if (wg < kPhaseMaxWg) g_phase[(size_t)wg * kPhaseSlots + kPhaseRecord] = 0;
     }
-    __device__ __forceinline__ void start() { t = __builtin_amdgcn_s_memrealtime(); }
+    __device__ __forceinline__ void start()
+    {
+        if (threadIdx.x == 0) phase_lds()[7] = __builtin_amdgcn_s_memrealtime();
+    }
     __device__ __forceinline__ void mark(int k)
     {
+        if (threadIdx.x != 0) return;
+        unsigned long long *s = phase_lds();
         const unsigned long long n = __builtin_amdgcn_s_memrealtime();
-        acc[k] += n - t;
-        t = n;
+        s[k] += n - s[7];
+        s[7] = n;
     }
     __device__ __forceinline__ void sync_mark(int k)
     {
         __syncthreads();
         mark(k);
     }
-    __device__ __forceinline__ void iter() { its++; }
+    __device__ __forceinline__ void iter()
+    {
+        if (threadIdx.x == 0) phase_lds()[kPhaseIters]++;
+    }
     __device__ __forceinline__ void flush(int wg)
     {
         if (threadIdx.x != 0 || wg >= kPhaseMaxWg) return;
-        for (int k = 0; k < kPhaseIters; k++) g_phase[(size_t)wg * kPhaseSlots + k] = acc[k];
-        g_phase[(size_t)wg * kPhaseSlots + kPhaseIters] = its;
+        for (int k = 0; k <= kPhaseIters; k++) g_phase[(size_t)wg * kPhaseSlots + k] = phase_lds()[k];
     }
 };
 __device__ __forceinline__ unsigned long long phase_now() { return __builtin_amdgcn_s_memrealtime(); }

@@ -849,6 +849,8 @@ extern "C" void mf_matrix_options_default(mf_matrix_options *o)
     o->keep_mixed = 1;
 }
 
+mf_matrix::~mf_matrix() { delete AT; }
+
 extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr, const int *colidx,
                                       const double *val, const mf_matrix_options *opt, void *stream,
                                       mf_matrix **out)
@@ -858,6 +860,8 @@ extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr,
     mf_matrix_options o;
     mf_matrix_options_default(&o);
     if (opt) o = *opt;
+    if (n_rows < 0 || n_cols < 0 || (n_rows > 0 && (!rowptr || rowptr[0] < 0 || rowptr[n_rows] < rowptr[0])))
+        return MF_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MF_ERR_NO_DEVICE;
     if (o.device < 0 || o.device >= ndev) return MF_ERR_INVALID;
@@ -869,6 +873,7 @@ extern "C" mf_status mf_matrix_create(int n_rows, int n_cols, const int *rowptr,
     A->keep_mixed = o.keep_mixed;
     if (n_rows >= 0 && rowptr) A->rowptr.assign(rowptr, rowptr + n_rows + 1);
     mf_status st = build_tiles(n_rows, n_cols, rowptr, colidx, A->ts);
+    if (st == MF_SUCCESS && n_rows > 0 && val) A->hval.assign(val, val + rowptr[n_rows]);
     if (st == MF_SUCCESS) {
         tile_classes(A->ts, rowptr, val, o.class_policy, A->cls);
         st = build_layout(A->ts, val, A->cls, o.keep_mixed != 0, s, A->L);
@@ -896,8 +901,58 @@ extern "C" mf_status mf_matrix_update_values(mf_matrix *A, const double *val, vo
     if (hipStreamSynchronize(s) != hipSuccess) return MF_ERR_HIP;
     tile_classes(A->ts, A->rowptr.data(), val, A->class_policy, A->cls);
     A->L.release();
+    if (A->ts.rows > 0) A->hval.assign(val, val + A->rowptr[A->ts.rows]);
+    delete A->AT;  // rebuilt from the new values on the next transposed call
+    A->AT = nullptr;
     return build_layout(A->ts, val, A->cls, A->keep_mixed != 0, s, A->L);
 }
+
+namespace mf {
+
+// CSR of A^T from the tile structure (entry e of tile t sits at row 16 bi + lrow[e], column
+// 16 tile_col[t] + lcol[e], CSR position src[e]): a counting sort by column over the rows
+// in order, so each row of A^T lists its columns (the rows of A) ascending -- the
+// reference's CSR transpose (utils.h:172) -- then the tiled matrix of that CSR.
+mf_status transposed(mf_matrix *A, hipStream_t s, mf_matrix **out)
+{
+    if (A->AT) {
+        *out = A->AT;
+        return MF_SUCCESS;
+    }
+    const TileStruct &ts = A->ts;
+    const int m = ts.rows, n = ts.cols;
+    if (m > 0 && A->hval.size() < (size_t)A->rowptr[m]) return MF_ERR_UNSUPPORTED;  // created without values
+    std::vector<int> col((size_t)(m > 0 ? A->rowptr[m] : 0), 0);
+    for (int bi = 0; bi < ts.tilem; bi++)
+        for (int64_t t = ts.tile_ptr[bi]; t < ts.tile_ptr[bi + 1]; t++)
+            for (int64_t e = ts.tile_off[t]; e < ts.tile_off[t + 1]; e++)
+                col[(size_t)ts.src[e]] = ts.tile_col[t] * kTile + ts.lcol[e];
+    std::vector<int> trp((size_t)n + 1, 0), tci((size_t)ts.nnz);
+    std::vector<double> tv((size_t)ts.nnz);
+    for (int i = 0; i < m; i++)
+        for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; j++) trp[(size_t)col[j] + 1]++;
+    for (int c = 0; c < n; c++) trp[(size_t)c + 1] += trp[c];
+    std::vector<int> fill(trp.begin(), trp.end() - 1);
+    for (int i = 0; i < m; i++)
+        for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; j++) {
+            const int d = fill[col[j]]++;
+            tci[d] = i;
+            tv[d] = A->hval[j];
+        }
+    mf_matrix_options o;
+    mf_matrix_options_default(&o);
+    o.device = A->device;
+    o.class_policy = A->class_policy;
+    o.keep_mixed = A->keep_mixed;
+    mf_matrix *T = nullptr;
+    const mf_status st = mf_matrix_create(n, m, trp.data(), tci.data(), tv.data(), &o, s, &T);
+    if (st != MF_SUCCESS) return st;
+    A->AT = T;
+    *out = T;
+    return MF_SUCCESS;
+}
+
+}  // namespace mf
 
 extern "C" mf_status mf_pattern_outer(mf_matrix *A, const double *d_u, const double *d_v, double scale,
                                       double *d_out, void *stream)
@@ -957,9 +1012,14 @@ extern "C" mf_status mf_spmv(mf_matrix *A, const double *d_x, double *d_y, int p
                              void *stream)
 {
     if (!A || !d_x || !d_y || !aligned16(d_x)) return MF_ERR_INVALID;
-    if (transpose) return MF_ERR_UNSUPPORTED;
     if (hipSetDevice(A->device) != hipSuccess) return MF_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
+    if (transpose) {
+        // y = A^T x: the SpMV of the transposed tiled matrix
+        mf_matrix *T = nullptr;
+        const mf_status st = transposed(A, s, &T);
+        return st != MF_SUCCESS ? st : mf_spmv(T, d_x, d_y, precision, 0, stream);
+    }
     const bool mixed = precision == MF_PREC_MIXED && A->L.has_lowp;
     int G = spmv_grid(A);
     const Partition *P;

@@ -548,7 +548,14 @@ extern "C" mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
-    if (o.transpose) return MF_ERR_UNSUPPORTED;
+    if (o.transpose) {
+        // A^T x = b: the same solve on the transposed tiled matrix
+        mf_matrix *T = nullptr;
+        const mf_status tst = transposed(A, (hipStream_t)stream, &T);
+        if (tst != MF_SUCCESS) return tst;
+        o.transpose = 0;
+        return mf_bicgstab_solve(T, d_b, d_x, &o, info, stream);
+    }
     const mf_status st = bicg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
     // beyond the persistent grid's capacity: the multi-kernel recurrence (info->variant = 0)
     if ((st == MF_ERR_TOO_LARGE || st == MF_ERR_OCCUPANCY) && o.grid == 0)

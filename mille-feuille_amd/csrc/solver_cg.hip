@@ -198,12 +198,12 @@ namespace mf {
 
 int default_grid(const mf_matrix *A, int maxG, int cg_rows)
 {
-    // single-system CG: cg_rows rows per workgroup (3072 tile-mixed, 1536 FP64), a
-    // multiple of 8 workgroups, at least 64; fewer, fuller workgroups make each grid
-    // reduction cheaper while the SpMV keeps pace (n = 262,144, tile-mixed: 2.66 ms on 96
-    // workgroups, 3.04 ms on 256; FP64: 3.09 ms on 192, 3.24 ms on 256).  The rule was
-    // measured on stencils; matrices with more than 16 entries per row keep the chunk rule
-    // below, whose bandwidth-bound SpMV wants every CU.
+    // single-system CG: cg_rows rows per workgroup, a multiple of 8 workgroups, at least
+    // 64.  With split block reductions a grid reduction costs about the same at 256
+    // workgroups as at 88, so smaller ranges win (variable-coefficient 3D diffusion 64^3,
+    // tile-mixed, value streams in LDS: 7.9 us per iteration on 256 workgroups, 8.1 on 176,
+    // 9.1 on 88).  The rule was measured on stencils; matrices with more than 16 entries
+    // per row keep the chunk rule below, whose bandwidth-bound SpMV wants every CU.
     if (cg_rows > 0 && A->L.nnz <= 16LL * A->L.n_rows) {
         long long g = ((int64_t)A->L.n_rows + cg_rows - 1) / cg_rows;
         g = (g + 7) / 8 * 8;
@@ -238,7 +238,7 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     if (A->L.tilem == 0) return MF_ERR_INVALID;
     // (pipelined CG holds five row vectors: at most 2 rows per thread, so the chunk rule)
     const int G = choose_grid(A, o, ((long long)(req ? req : 2) << 8) | (mixed ? 1 : 0),
-                              dyn || req == 3 ? 0 : (mixed ? 3072 : 1536));
+                              dyn || req == 3 ? 0 : 1024);
     if (G > A->L.tilem) return MF_ERR_INVALID;
     const Partition *P = nullptr;
     mf_status st = MF_SUCCESS;
@@ -255,10 +255,14 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
     // registers.
     const size_t lds_x = (64 + (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     // Default: sliced-ELL for tile-mixed solves (its row-per-lane SpMV halves the mixed
-    // SpMV time) and for matrices with few entries per tile; tile chunks for the other
-    // all-FP64 solves, whose streams are bandwidth-bound and where the chunks carry less
-    // padding.  layout 1 / 2 force chunks / sliced-ELL.
-    const bool want_sell = o->layout == 2 || (o->layout == 0 && (mixed || sparse_tiles(A)));
+    // SpMV time), for matrices with few entries per tile, and for all-FP64 solves whose
+    // binary64 value stream can stay in LDS (below; 2D Poisson 64 x 64: 5.1 us per
+    // iteration against 6.2 on tile chunks); tile chunks for the other all-FP64 solves,
+    // whose streams are bandwidth-bound and where the chunks carry less padding.  layout
+    // 1 / 2 force chunks / sliced-ELL.
+    const bool fp64_small = o->layout == 0 && !mixed && !dyn && o->value_cache == 0 && !sparse_tiles(A) &&
+                            (double)A->L.nnz * 8.0 * 1.5 / G + (double)lds_x + 4096.0 <= (double)A->max_lds;
+    const bool want_sell = o->layout == 2 || (o->layout == 0 && (mixed || sparse_tiles(A) || fp64_small));
     bool sell = want_sell && !dyn && sell_supported(A, rpt) && lds_x <= (size_t)A->max_lds;
     size_t lds = sell ? lds_x : (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
     bool xreg = false;
@@ -305,8 +309,10 @@ mf_status cg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_solve
             on_chip = true;
         }
     }
-    if (sell && !k && dsell.perm) {
-        // the partition's streams were built permuted for another variant: tile chunks
+    if (sell && ((!k && dsell.perm) || (fp64_small && var == 4 && !on_chip))) {
+        // the partition's streams were built permuted for another variant, or an all-FP64
+        // stream chosen for LDS residency did not fit: tile chunks
+        on_chip = false;
         sell = false;
         mixed_arith = mixed;
         lds = (64 + 2 * (size_t)std::max(P->rows_cap, 1)) * sizeof(double);
@@ -385,7 +391,14 @@ extern "C" mf_status mf_cg_solve(mf_matrix *A, const double *d_b, double *d_x, c
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
-    if (o.transpose) return MF_ERR_UNSUPPORTED;
+    if (o.transpose) {
+        // A^T x = b: the same solve on the transposed tiled matrix
+        mf_matrix *T = nullptr;
+        const mf_status tst = transposed(A, (hipStream_t)stream, &T);
+        if (tst != MF_SUCCESS) return tst;
+        o.transpose = 0;
+        return mf_cg_solve(T, d_b, d_x, &o, info, stream);
+    }
     const mf_status st = cg_launch(A, d_b, d_x, &o, info, (hipStream_t)stream);
     // a system the persistent grid cannot hold (more rows than co-resident workgroups
     // keep in registers, e.g. beyond 2,097,152 rows on 256 CUs) runs the multi-kernel
@@ -571,7 +584,12 @@ extern "C" mf_status mf_cg_solve_noalloc(mf_matrix *A, const double *d_b, double
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
-    if (o.transpose) return MF_ERR_UNSUPPORTED;
+    if (o.transpose) {
+        // the transposed matrix must exist already (an earlier transposed call built it)
+        if (!A->AT) return MF_ERR_UNSUPPORTED;
+        A = A->AT;
+        o.transpose = 0;
+    }
     unsigned char *base = nullptr;
     mf_status st = na_scratch(A->device, &base);
     if (st != MF_SUCCESS) return st;

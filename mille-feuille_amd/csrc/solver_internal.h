@@ -182,9 +182,20 @@ struct mf_matrix {
     mf::Workspace ws;
     std::map<long long, int> tuned_grid;  // autotuner cache: key -> workgroups
     int tuned_variant[2] = {0, 0};        // autotuner cache: fp64 / mixed CG variant (0 = default)
+    std::vector<double> hval;             // host copy of the CSR values (input positions)
+    mf_matrix *AT = nullptr;              // A^T, built on the first transposed call
+    mf_matrix() = default;
+    mf_matrix(const mf_matrix &) = delete;
+    mf_matrix &operator=(const mf_matrix &) = delete;
+    ~mf_matrix();
 };
 
 namespace mf {
+// A^T as a tiled matrix of its own (same device, class policy and value copies), built
+// from the host copy of the pattern and values on the first call and kept until the
+// values change.
+mf_status transposed(mf_matrix *A, hipStream_t s, mf_matrix **out);
+
 // Tile chunks carry a 12-byte header per chunk of up to 16 entries: with few entries per
 // tile they cost more per entry than the sliced-ELL column offsets.
 inline bool sparse_tiles(const mf_matrix *A) { return A->L.tiles > 0 && A->L.nnz < 8 * A->L.tiles; }

@@ -438,6 +438,11 @@ extern "C" mf_status mf_cg_solve_multikernel(mf_matrix *A, const double *d_b, do
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    if (o.transpose) {
+        mf_status st = transposed(A, (hipStream_t)stream, &A);
+        if (st != MF_SUCCESS) return st;
+        o.transpose = 0;
+    }
     return cg_multikernel(A, d_b, d_x, &o, info, (hipStream_t)stream);
 }
 
@@ -449,5 +454,10 @@ extern "C" mf_status mf_bicgstab_solve_multikernel(mf_matrix *A, const double *d
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    if (o.transpose) {
+        mf_status st = transposed(A, (hipStream_t)stream, &A);
+        if (st != MF_SUCCESS) return st;
+        o.transpose = 0;
+    }
     return bicg_multikernel(A, d_b, d_x, &o, info, (hipStream_t)stream);
 }

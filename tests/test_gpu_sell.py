@@ -139,7 +139,8 @@ def test_wide_offsets_fall_back_to_chunks(mf, orc, torch):
 def test_unit_profiles(mf, orc, torch):
     """Rows spread over two classes (convection-diffusion: binary16 and int8 tiles) take the
     narrow unit profile, single-class rows (Poisson: int8) the wide one; both reach the
-    FP64 tolerance, and the layout and variant used are reported."""
+    FP64 tolerance, and the layout and variant used are reported.  A small all-FP64 solve
+    keeps its binary64 stream in LDS."""
     rp, ci, va = mf.workloads.convdiff3d(16)
     b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
     A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
@@ -156,7 +157,12 @@ def test_unit_profiles(mf, orc, torch):
     A = mf.TiledMatrix(rp, ci, va, class_policy="lossless")
     x, info = A.cg(b, tol=1e-10, precision="mixed")
     assert info["status"] == 0 and info["layout"] in (2, 6) and info["variant"] == 4, info
+    # an all-FP64 stream this small stays in LDS: sliced-ELL with on-chip values; without
+    # the value cache the all-FP64 solve takes the tile chunks
     x, info = A.cg(b, tol=1e-10, precision="fp64")
+    assert info["status"] == 0 and info["layout"] == 6 and info["variant"] == 4, info
+    assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    x, info = A.cg(b, tol=1e-10, precision="fp64", value_cache=False)
     assert info["status"] == 0 and info["layout"] == 1 and info["variant"] == 4, info
     A.close()
 

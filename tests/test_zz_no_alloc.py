@@ -100,3 +100,23 @@ def test_noalloc_refuses_what_it_cannot_hold(mf, torch):
         assert mf._lib.STATUS_NAMES[st] == "MF_ERR_TOO_LARGE"
     finally:
         D.close()
+
+
+def test_noalloc_transposed_after_transposed_solve(mf, torch):
+    """With transpose set the no-alloc entry point runs on the transposed matrix, which an
+    earlier transposed mf_cg_solve must have built (it allocates)."""
+    print("noalloc: transposed", flush=True)
+    rp, ci, va = mf.workloads.poisson3d(20)
+    b, _ = mf.workloads.rhs_from_solution(rp, ci, va, seed=3)
+    A = mf.TiledMatrix(rp, ci, va)
+    try:
+        st, _, _ = _solve(mf, torch, "mf_cg_solve_noalloc", A, b, precision="mixed", transpose=True)
+        assert mf._lib.STATUS_NAMES[st] == "MF_ERR_UNSUPPORTED"
+        st0, x0, i0 = _solve(mf, torch, "mf_cg_solve", A, b, precision="mixed", transpose=True)
+        assert st0 == 0, i0
+        st1, x1, i1 = _solve(mf, torch, "mf_cg_solve_noalloc", A, b, precision="mixed", transpose=True)
+        assert st1 == 0, i1
+        assert i1["layout"] == i0["layout"] and i1["layout"] != 1
+        assert i1["iterations"] == i0["iterations"] and np.array_equal(x1, x0)
+    finally:
+        A.close()
