@@ -358,11 +358,15 @@ mf_status mf_batch_create(int count, const int *n_rows, const int *const *rowptr
 mf_status mf_batch_destroy(mf_batch *B);
 mf_status mf_batch_row_offsets(const mf_batch *B, int64_t *offsets /* count+1 */);
 /* One persistent launch, per-system barrier groups, early retirement.  When
- * use_graph != 0 the launch is captured once into a hipGraph and replayed. */
+ * use_graph != 0 the launch is captured once into a hipGraph and replayed.  The SpMV
+ * reads the sliced-ELL streams of the member ranges (per-system info layout 2/3) unless
+ * opt->layout = 1 or a system's columns lie beyond the streams' 16-bit offsets (tile
+ * chunks, layout 1). */
 mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
                               int use_graph, mf_solve_info *per_system, double *time_ms, void *stream);
-/* Bound on the rows one workgroup of a system holds (1024, 2048, 4096 or 8192): systems
- * larger than that are split over 2, 4 or 8 workgroups with their own barrier group. */
+/* Bound on the rows one workgroup of a system holds (1024, 2048, 4096 or 8192; default
+ * 4096): systems larger than that are split over 2, 4, ..., 32 workgroups with their own
+ * barrier group. */
 mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
 /* Autotuner of the batched launch (the reference's analogue is the partition search over
  * nnz_list / block_per_warp, main-cg.cu:654-700): times `iterations`-iteration
@@ -372,6 +376,10 @@ mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
  * reports it.  The trial iterates go to an internal vector; d_b is only read. */
 mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int precision, int iterations, int *best_rows_per_wg,
                             int *best_unroll, double *best_ms, void *stream);
+/* Diagnostics (no counterpart in the reference): each system's solve interval in the last
+ * batched launch, in ms from the earliest system start (the kernel's 100 MHz clock), and
+ * the barrier-group size (workgroups) it ran with; arrays of count entries. */
+mf_status mf_batch_timeline(const mf_batch *B, double *start_ms, double *end_ms, int *group_size);
 
 /* Diagnostics (no counterpart in the reference).  Per-phase timers of the last
  * variant-4 single-kernel CG launch: n_wg x 8 counters per logical workgroup, in

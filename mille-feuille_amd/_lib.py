@@ -153,6 +153,7 @@ _SIGS = {
                                            ctypes.POINTER(SolveInfo), f64p, vp]),
     "mf_batch_set_groups": (ctypes.c_int, [vp, ctypes.c_int]),
     "mf_batch_autotune": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, i32p, i32p, f64p, vp]),
+    "mf_batch_timeline": (ctypes.c_int, [vp, f64p, f64p, i32p]),
     "mf_phase_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
     "mf_device_count": (ctypes.c_int, []),
     "mf_device_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),

@@ -406,6 +406,14 @@ class Batch:
               "mf_batch_autotune")
         return r.value, u.value, t.value
 
+    def timeline(self):
+        """Per-system (start_ms, end_ms, group size) of the last batched launch."""
+        st, en = np.zeros(self.count), np.zeros(self.count)
+        gs = np.zeros(self.count, np.int32)
+        check(L.lib().mf_batch_timeline(self.h, st.ctypes.data_as(L.f64p), en.ctypes.data_as(L.f64p),
+                                        gs.ctypes.data_as(L.i32p)), "mf_batch_timeline")
+        return st, en, gs
+
     def pack(self, vectors):
         """Concatenate per-system vectors at the padded offsets (zeros in the padding)."""
         torch = _torch()

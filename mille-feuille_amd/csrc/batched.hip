@@ -3,14 +3,15 @@
 //
 // The systems are laid out as one block-diagonal tiled matrix, every system's rows
 // starting at a multiple of 16 so that a tile row belongs to one system.  Systems are
-// binned by size into pools; a pool's workgroups form barrier groups of W (1, 2, 4
-// or 8) workgroups, W the smallest power of two that keeps a member at or below the
+// binned by size into pools; a pool's workgroups form barrier groups of W (1, 2, 4, ...,
+// kMaxGroup) workgroups, W the smallest power of two that keeps a member at or below the
 // tuned rows-per-workgroup bound.  Each group pops systems from its pool's queue
-// (largest first) and solves them one after another with the CG core: W = 1 groups
-// synchronise with block barriers only, larger groups with flagged records among
-// their members.  A converged system retires its group to the next one in the same
-// launch.  The launch (queue reset + kernel) can be captured once into a hipGraph
-// and replayed.
+// (largest first) and solves them one after another with the variant-4 CG core: one
+// reduction per iteration among the group's members (within the workgroup when W = 1)
+// and point-to-point publishes of p through the member ranges' flags.  The SpMV reads
+// the sliced-ELL streams of the member ranges.  A converged system retires its group to
+// the next one in the same launch.  The launch (reset + kernel) can be captured once
+// into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,17 +49,36 @@ struct BatchArgs {
     double *info;               // 8 doubles per system
     double tol2, delta2;
     int maxit, fixed, need64, need32, max_repl;
+    DevSell S;                  // sliced-ELL streams of the member ranges (SELL kernels)
+    NbSync nb;                  // publish flags and dependency lists of the member ranges
 };
 
-__global__ void batch_reset_kernel(unsigned long long *recs, long long n, unsigned *ctl, unsigned *queue, int npools)
+// Before each batched launch: records, abort flag and queues, and -- for the sliced-ELL
+// kernels -- the publish buffers and x over every row of the block-diagonal matrix.  A
+// sliced-ELL row's zero-valued padding entries gather p (and, in residual checks, x) at
+// rows of other systems or between systems; those must hold finite values, since
+// 0 * NaN is NaN (every solve starts from x = 0, and writes x for its own rows).
+__global__ void batch_reset_kernel(unsigned long long *recs, long long n, unsigned *ctl, unsigned *queue, int npools,
+                                   unsigned *flags, long long nflags, double *p64, float *p32, double *x,
+                                   long long nrows)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         recs[i] = 0ull;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nflags;
+         i += (long long)gridDim.x * blockDim.x)
+        flags[i] = 0u;
+    if (p64 || p32 || x)
+        for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nrows;
+             i += (long long)gridDim.x * blockDim.x) {
+            if (p64) p64[i] = 0.0;
+            if (p32) p32[i] = 0.0f;
+            if (x) x[i] = 0.0;
+        }
     if (blockIdx.x == 0 && threadIdx.x < 64) ctl[threadIdx.x] = 0u;
     if (blockIdx.x == 0 && threadIdx.x < npools) queue[threadIdx.x] = 0u;
 }
 
-template <int RPT, bool MIXED, int U, bool SINGLE>
+template <int RPT, bool MIXED, int U, bool SINGLE, int SELL>
 __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, int g0, double *scratch, double *q,
                              double *xs)
 {
@@ -69,10 +89,9 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
     gsy.gs.ctl = a.ctl;
     gsy.gs.timeout = a.timeout;
     gsy.gs.G = W;
-    gsy.gs.acq = 1;  // tile-chunk gathers are plain loads
+    gsy.gs.acq = SELL ? 0 : 1;  // tile-chunk gathers are plain loads, sliced-ELL gathers agent-scope
     gsy.wg = member;
     gsy.epoch = 0;
-    BlockOps bsy;
     for (;;) {
         int idx;
         if (SINGLE) {
@@ -87,6 +106,7 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
             idx = (int)v[0];
         }
         if (idx >= nsys) return;
+        const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
         const int s = a.order[a.pool_sys0[pool] + idx];
         const int t0 = a.sys_tr[s], t1 = a.sys_tr[s + 1];
         const int mt0 = t0 + (int)((long long)(t1 - t0) * member / W);
@@ -113,9 +133,14 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
         part.dvis = nullptr;
         part.dyn_mode = 0;
         part.n_sys = a.sys_rows[s];
-        CgResult res;
-        if (SINGLE) res = cg_core<RPT, 2, MIXED, 1024, BlockOps, U>(part, R, bsy, scratch, q, xs);
-        else res = cg_core<RPT, 2, MIXED, 1024, GridOps, U>(part, R, gsy, scratch, q, xs);
+        part.S = a.S;
+        // this wave's slot in the member range's streams
+        part.slot = __builtin_amdgcn_readfirstlane((a.sys_rng0[s] + member) * 16 + (int)(threadIdx.x >> 6));
+        part.nb = a.nb;
+        part.nb_id = a.sys_rng0[s] + member;
+        // variant 4: one group reduction per iteration, p published point to point to the
+        // members whose rows gather it (a group of one reduces within the workgroup)
+        const CgResult res = cg_core_nb<RPT, MIXED, 1024, U, false, SELL>(part, R, gsy, scratch, q, xs);
         if (member == 0 && threadIdx.x == 0) {
             double *inf = a.info + (size_t)s * 8;
             inf[0] = (double)res.it;
@@ -124,13 +149,17 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
             inf[3] = res.rt;
             inf[4] = (double)res.status;
             inf[5] = (double)res.repl;
+            // the system's solve interval in s_memrealtime ticks (100 MHz): mf_batch_timeline
+            inf[6] = (double)t_start;
+            inf[7] = (double)__builtin_amdgcn_s_memrealtime();
         }
         if (res.status == ST_TIMEOUT) return;
         __syncthreads();
     }
 }
 
-template <int RPT, bool MIXED, int U>
+// SELL: 0 tile chunks; 1/2 sliced-ELL streams of the member ranges, wide/narrow profile
+template <int RPT, bool MIXED, int U, int SELL = 0>
 __global__ __launch_bounds__(1024) void batched_cg_kernel(BatchArgs a)
 {
     extern __shared__ double smem[];
@@ -145,8 +174,8 @@ __global__ __launch_bounds__(1024) void batched_cg_kernel(BatchArgs a)
     const int local = b - a.pool_wg0[pool];
     const int member = local % W;
     const int g0 = b - member;
-    if (W == 1) batch_worker<RPT, MIXED, U, true>(a, pool, W, member, g0, scratch, q, xs);
-    else batch_worker<RPT, MIXED, U, false>(a, pool, W, member, g0, scratch, q, xs);
+    if (W == 1) batch_worker<RPT, MIXED, U, true, SELL>(a, pool, W, member, g0, scratch, q, xs);
+    else batch_worker<RPT, MIXED, U, false, SELL>(a, pool, W, member, g0, scratch, q, xs);
 }
 
 }  // namespace mfk
@@ -157,7 +186,10 @@ struct mf_batch {
     std::vector<int> rows, tr0;        // per system
     std::vector<int64_t> row_off;      // count+1 (padded offsets)
     std::vector<int64_t> nnz;
-    int rows_per_wg = 8192;            // bound on a member's rows (mf_batch_set_groups / autotuner)
+    // bound on a member's rows (mf_batch_set_groups / autotuner).  4096 rows (4 per thread)
+    // by default: at 8 rows per thread the batched kernels exceed the register budget (the
+    // tile-mixed sliced-ELL kernel spills 151 VGPRs, 1.3 KB of scratch per lane)
+    int rows_per_wg = 4096;
     int unroll = 0;                    // chunk groups (16 tile chunks each) per wave per SpMV batch;
                                        // 0: the register-budget default (spmv_unroll)
     // device tables
@@ -302,17 +334,21 @@ extern "C" mf_status mf_batch_set_groups(mf_batch *B, int rows_per_wg_max)
 
 namespace mf {
 
+// Barrier-group sizes 1, 2, 4, ..., kMaxGroup workgroups per system.
+constexpr int kGroupClasses = 6;
+constexpr int kMaxGroup = 1 << (kGroupClasses - 1);
+
 // Size classes -> pools of workgroups proportional to each class's nonzeros.
 static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
 {
     if (B->planned) return MF_SUCCESS;
     const int R = B->rows_per_wg;
-    std::vector<std::vector<int>> cls(4);
+    std::vector<std::vector<int>> cls(kGroupClasses);
     std::vector<int> sysW(B->count);
     for (int sidx = 0; sidx < B->count; sidx++) {
         const int tiles_rows = (int)((B->row_off[sidx + 1] - B->row_off[sidx]) / 16);
         int W = 1, ci = 0;
-        while (ci < 3 && (long long)(tiles_rows + W - 1) / W * 16 > R) {
+        while (ci < kGroupClasses - 1 && (long long)(tiles_rows + W - 1) / W * 16 > R) {
             W *= 2;
             ci++;
         }
@@ -336,7 +372,7 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
     std::vector<int> pW, pwg0, psys0, order;
     std::vector<double> work;
     double total = 0;
-    for (int ci = 0; ci < 4; ci++) {
+    for (int ci = 0; ci < kGroupClasses; ci++) {
         if (cls[ci].empty()) continue;
         auto &v = cls[ci];
         std::sort(v.begin(), v.end(), [&](int a, int b) { return B->nnz[a] > B->nnz[b]; });
@@ -398,6 +434,25 @@ static BatchKernel pick_batch_u(int rpt)
     }
 }
 
+// The sliced-ELL kernels (SELL 1 wide, 2 narrow): the chunk-group count does not apply.
+template <bool MIXED, int SELL>
+static BatchKernel pick_batch_sell(int rpt)
+{
+    switch (rpt) {
+    case 1: return mfk::batched_cg_kernel<1, MIXED, 1, SELL>;
+    case 2: return mfk::batched_cg_kernel<2, MIXED, 1, SELL>;
+    case 4: return mfk::batched_cg_kernel<4, MIXED, 1, SELL>;
+    case 8: return mfk::batched_cg_kernel<8, MIXED, 1, SELL>;
+    default: return nullptr;
+    }
+}
+
+static BatchKernel pick_batch_sell(int rpt, bool mixed, bool narrow)
+{
+    if (mixed) return narrow ? pick_batch_sell<true, 2>(rpt) : pick_batch_sell<true, 1>(rpt);
+    return pick_batch_sell<false, 1>(rpt);
+}
+
 template <bool MIXED>
 static BatchKernel pick_batch(int rpt, int unroll)
 {
@@ -438,7 +493,29 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     mf_status st = batch_plan(B, Gmax, s);
     if (st != MF_SUCCESS) return st;
     if (B->G > occ * A->num_cus) return MF_ERR_OCCUPANCY;
-    if ((st = A->ws.ensure(A->L.n_rows, B->G, s)) != MF_SUCCESS) return st;
+    // Sliced-ELL streams of the member ranges (as the single-system CG): rows in lanes,
+    // no chunk headers, agent-scope gathers.  layout 1 keeps the tile chunks.
+    mfk::DevSell dsell;
+    memset(&dsell, 0, sizeof(dsell));
+    if (o.layout != 1 && sell_supported(A, rpt)) {
+        bool narrow = false;
+        if (ensure_sell(A, &B->part, rpt, mixed, s, &dsell, &narrow, false, true) == MF_SUCCESS) {
+            BatchKernel ks = pick_batch_sell(rpt, mixed, narrow);
+            int occ2 = 0;
+            if (ks &&
+                hipFuncSetAttribute((const void *)ks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                    hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, ks, 1024, lds) == hipSuccess &&
+                B->G <= occ2 * A->num_cus)
+                k = ks;
+            else
+                memset(&dsell, 0, sizeof(dsell));
+        } else {
+            memset(&dsell, 0, sizeof(dsell));
+        }
+    }
+    // records per launched workgroup, publish flags per member range
+    if ((st = A->ws.ensure(A->L.n_rows, std::max(B->G, B->part.G), s)) != MF_SUCCESS) return st;
     mfk::BatchArgs a;
     memset(&a, 0, sizeof(a));  // the graph key compares the bytes, padding included
     a.A = A->L.dev(B->part);
@@ -468,6 +545,13 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     a.need64 = mixed ? 0 : 1;  // tile-mixed products take the binary32 p (kPmLow32)
     a.need32 = mixed ? 1 : 0;
     a.max_repl = o.max_replacements > 0 ? o.max_replacements : 64;
+    a.S = dsell;
+    a.nb.flags = A->ws.flags;
+    a.nb.dep_ptr = B->part.dep_ptr;
+    a.nb.dep_idx = B->part.dep_idx;
+    const long long nflags = (long long)mfk::kFlagStride * B->part.G;
+    const bool clear = dsell.s[mfk::kSellAll64].soff != nullptr;  // sliced-ELL: finite p and x everywhere
+    const int reset_grid = clear ? 1024 : 1;
     const long long nrec = (long long)mfk::kRecWordsPerMember * B->G;
     struct GraphKey {
         mfk::BatchArgs a;
@@ -505,8 +589,10 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
                 (void)hipStreamDestroy(cs);
                 return done(MF_ERR_HIP);
             }
-            hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, cs, A->ws.recs, nrec, A->ws.ctl,
-                               B->d_queue, B->npools);
+            hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(reset_grid), dim3(256), 0, cs, A->ws.recs, nrec, A->ws.ctl,
+                               B->d_queue, B->npools, A->ws.flags, nflags, clear ? A->ws.p64 : nullptr,
+                               clear ? A->ws.p32 : nullptr,
+                               clear ? d_x : nullptr, (long long)A->L.n_rows);
             hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, cs, a);
             const hipError_t ce = hipStreamEndCapture(cs, &B->graph);
             (void)hipStreamDestroy(cs);
@@ -525,8 +611,10 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
         }
         if (hipGraphLaunch(B->gexec, s) != hipSuccess) return done(MF_ERR_HIP);
     } else {
-        hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(1), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl,
-                           B->d_queue, B->npools);
+        hipLaunchKernelGGL(mfk::batch_reset_kernel, dim3(reset_grid), dim3(256), 0, s, A->ws.recs, nrec, A->ws.ctl,
+                           B->d_queue, B->npools, A->ws.flags, nflags, clear ? A->ws.p64 : nullptr,
+                           clear ? A->ws.p32 : nullptr,
+                           clear ? d_x : nullptr, (long long)A->L.n_rows);
         hipLaunchKernelGGL(k, dim3(B->G), dim3(1024), lds, s, a);
         if (hipGetLastError() != hipSuccess) return done(MF_ERR_HIP);
     }
@@ -558,7 +646,8 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
             in.grid = B->G;
             in.block = 1024;
             in.rows_per_thread = rpt;
-            in.layout = 1;  // tile chunks
+            // 1 tile chunks, 2/3 sliced-ELL wide/narrow (as mf_solve_info of a single solve)
+            in.layout = a.S.s[mfk::kSellAll64].soff ? (mixed && k == pick_batch_sell(rpt, true, true) ? 3 : 2) : 1;
             in.precision = mixed ? MF_PREC_MIXED : MF_PREC_FP64;
         }
     }
@@ -595,7 +684,7 @@ extern "C" mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int preci
                 if (st != MF_SUCCESS) break;
                 t = std::min(t, ms);
             }
-            if (st != MF_SUCCESS) {  // e.g. a system beyond r rows per member on 8 members
+            if (st != MF_SUCCESS) {  // e.g. a system beyond r rows per member on kMaxGroup members
                 st = MF_SUCCESS;
                 continue;
             }
@@ -619,5 +708,29 @@ extern "C" mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int preci
     if (best_rows_per_wg) *best_rows_per_wg = br;
     if (best_unroll) *best_unroll = bu;
     if (best_ms) *best_ms = best;
+    return MF_SUCCESS;
+}
+
+// Per-system solve intervals of the last batched launch, in ms from the earliest system
+// start (s_memrealtime, 100 MHz), and the barrier-group size each system ran with.
+extern "C" mf_status mf_batch_timeline(const mf_batch *B, double *start_ms, double *end_ms, int *group_size)
+{
+    if (!B || !start_ms || !end_ms) return MF_ERR_INVALID;
+    if (hipSetDevice(B->M->device) != hipSuccess) return MF_ERR_HIP;
+    std::vector<double> h((size_t)8 * B->count);
+    if (hipMemcpy(h.data(), B->d_info, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return MF_ERR_HIP;
+    double t0 = 1e300;
+    for (int i = 0; i < B->count; i++) t0 = std::min(t0, h[(size_t)8 * i + 6]);
+    for (int i = 0; i < B->count; i++) {
+        start_ms[i] = (h[(size_t)8 * i + 6] - t0) * 1e-5;
+        end_ms[i] = (h[(size_t)8 * i + 7] - t0) * 1e-5;
+        if (group_size) {
+            const int tiles_rows = (int)((B->row_off[i + 1] - B->row_off[i]) / 16);
+            int W = 1;
+            while (W < mf::kMaxGroup && (long long)(tiles_rows + W - 1) / W * 16 > B->rows_per_wg) W *= 2;
+            group_size[i] = W;
+        }
+    }
     return MF_SUCCESS;
 }

@@ -77,6 +77,7 @@ struct CgPart {
     DevSell S;       // sliced-ELL streams (SELL kernels) and this wave's slot in them
     int slot;
     NbSync nb;       // publish flags and dependency lists (variant 4)
+    int nb_id;       // this workgroup's range: its publish flag and dependency list
 };
 
 // This workgroup's share of x: in LDS (XREG = false) or in registers (XREG = true).
@@ -508,7 +509,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
         sell_prefetch<sell_lead_stream<MIXED>(), RPT, NAR, PM, LDSV>(a.S.s[sell_lead_stream<MIXED>()], a.slot, lead);
     while (alive && res.status == ST_NOTCONV) {
         pc.start();
-        if (!p_visible && !nb_wait(sy.gs, a.nb, sy.wg, pub)) {
+        if (!p_visible && !nb_wait(sy.gs, a.nb, a.nb_id, pub)) {
             res.status = ST_TIMEOUT;
             break;
         }
@@ -583,7 +584,7 @@ __device__ CgResult cg_core_nb(const CgPart &a, Rows<RPT, BLK> &R, GridOps &sy, 
             p[k] = r[k] + beta * p[k];
         }
         publish_p<RPT, BLK>(a, R, p);
-        nb_signal(a.nb, sy.wg, ++pub);
+        nb_signal(a.nb, a.nb_id, ++pub);
         res.it++;
         pc.mark(4);
         pc.iter();
@@ -890,6 +891,7 @@ __global__ __launch_bounds__(BLK) void cg_kernel(CgArgs a)
     part.S = a.S;
     part.slot = __builtin_amdgcn_readfirstlane(wg * (BLK / 64) + (int)(threadIdx.x >> 6));
     part.nb = a.nb;
+    part.nb_id = wg;
     CgResult res;
     if constexpr (VAR == 4)
         res = cg_core_nb<RPT, MIXED, BLK, spmv_unroll<RPT, XREG ? 3 : 2>(), XREG, SELL>(part, R, sy, scratch, q, xs);
