@@ -373,7 +373,10 @@ mf_status mf_batch_set_groups(mf_batch *B, int wg_per_system_max);
  * fixed-budget solves of every system (default 30) for each bound on the rows per
  * workgroup (1024..8192, i.e. workgroups per system) and 1 or 2 chunk groups (16 tile
  * chunks) per wave per SpMV batch, keeps the fastest in the batch for later solves and
- * reports it.  The trial iterates go to an internal vector; d_b is only read. */
+ * reports it.  It then solves the batch once to the tolerance (1e-10) with that shape,
+ * takes each system's measured cost (group size x solve time) and keeps the plan that
+ * shares the workgroups among the group-size pools by those costs when its launch is
+ * faster.  The trial iterates go to an internal vector; d_b is only read. */
 mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int precision, int iterations, int *best_rows_per_wg,
                             int *best_unroll, double *best_ms, void *stream);
 /* Diagnostics (no counterpart in the reference): each system's solve interval in the last

@@ -200,6 +200,11 @@ struct mf_batch {
     unsigned *d_queue = nullptr;
     double *d_info = nullptr;
     int npools = 0, G = 0;
+    // measured cost of each system (workgroups x its solve time in a launch to the
+    // tolerance, mf_batch_autotune) for the plan of rows_per_wg = cost_rows; empty: the
+    // plan shares workgroups by nonzeros and rows
+    std::vector<double> sys_cost;
+    int cost_rows = 0;
     bool planned = false;
     // captured launch
     hipGraphExec_t gexec = nullptr;
@@ -377,7 +382,8 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
         auto &v = cls[ci];
         std::sort(v.begin(), v.end(), [&](int a, int b) { return B->nnz[a] > B->nnz[b]; });
         double w = 0;
-        for (int x : v) w += (double)B->nnz[x] + (double)B->rows[x];
+        const bool measured = B->cost_rows == R && (int)B->sys_cost.size() == B->count;
+        for (int x : v) w += measured ? B->sys_cost[x] : (double)B->nnz[x] + (double)B->rows[x];
         pW.push_back(1 << ci);
         work.push_back(w);
         total += w;
@@ -403,6 +409,24 @@ static mf_status batch_plan(mf_batch *B, int Gmax, hipStream_t s)
         if (nwg[p] <= pW[p]) return MF_ERR_OCCUPANCY;
         nwg[p] -= pW[p];
         used -= pW[p];
+    }
+    // the workgroups the rounding left over: one more group at a time to the pool with the
+    // most work per group (that can still use one)
+    for (;;) {
+        int best = -1;
+        double load = 0;
+        for (int p = 0; p < np; p++) {
+            const int nsys = psys0[p + 1] - psys0[p];
+            if (used + pW[p] > Gmax || nwg[p] + pW[p] > pW[p] * nsys) continue;
+            const double l = work[p] / (nwg[p] / pW[p]);
+            if (best < 0 || l > load) {
+                best = p;
+                load = l;
+            }
+        }
+        if (best < 0) break;
+        nwg[best] += pW[best];
+        used += pW[best];
     }
     pwg0.push_back(0);
     for (int p = 0; p < np; p++) pwg0.push_back(pwg0.back() + nwg[p]);
@@ -695,8 +719,8 @@ extern "C" mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int preci
             }
         }
     }
-    mf::free_dev(dx);
     if (best >= 1e30) {
+        mf::free_dev(dx);
         if (rows0 != B->rows_per_wg) B->free_plan();
         B->rows_per_wg = rows0;
         B->unroll = unroll0;
@@ -705,6 +729,34 @@ extern "C" mf_status mf_batch_autotune(mf_batch *B, const double *d_b, int preci
     if (br != B->rows_per_wg) B->free_plan();
     B->rows_per_wg = br;
     B->unroll = bu;
+    // Pool shares from measured costs.  Solve times are not proportional to nonzeros (the
+    // iteration counts differ and small systems pay fixed latencies), so one launch to the
+    // tolerance with the chosen shape measures every system's cost -- its group size times
+    // its solve interval -- and the plan that shares workgroups by those costs is kept when
+    // its launch is faster.
+    if (!B->sys_cost.empty()) B->free_plan();
+    B->sys_cost.clear();
+    B->cost_rows = 0;
+    mf_solve_options of;
+    mf_solve_options_default(&of);
+    of.precision = precision;
+    double t_plain = 0, t_cost = 0;
+    if (mf_cg_solve_batched(B, d_b, dx, &of, 0, nullptr, &t_plain, stream) == MF_SUCCESS) {
+        std::vector<double> t0v(B->count), t1v(B->count);
+        std::vector<int> gw(B->count);
+        if (mf_batch_timeline(B, t0v.data(), t1v.data(), gw.data()) == MF_SUCCESS) {
+            B->sys_cost.resize(B->count);
+            for (int i = 0; i < B->count; i++) B->sys_cost[i] = gw[i] * std::max(1e-6, t1v[i] - t0v[i]);
+            B->cost_rows = br;
+            B->free_plan();
+            if (mf_cg_solve_batched(B, d_b, dx, &of, 0, nullptr, &t_cost, stream) != MF_SUCCESS || t_cost >= t_plain) {
+                B->sys_cost.clear();
+                B->cost_rows = 0;
+                B->free_plan();
+            }
+        }
+    }
+    mf::free_dev(dx);
     if (best_rows_per_wg) *best_rows_per_wg = br;
     if (best_unroll) *best_unroll = bu;
     if (best_ms) *best_ms = best;
