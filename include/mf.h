@@ -206,7 +206,9 @@ typedef struct mf_solve_options {
     int dynamic_precision; /* 0 off; 1 partial-convergence demotion (|dp| relative to ||p||_inf);
                               2 the reference's absolute |dp| thresholds */
     double replace_delta;  /* mixed: replace r by b - A x once ||r|| < delta * ||r|| at the last
-                              replacement (0 = default 1e-3) */
+                              replacement (0 = the solver's default: for the single-kernel and
+                              batched CG 1e-5 under lossless classes, else 1e-3; 1e-3 for
+                              BiCGSTAB and the multi-kernel solvers) */
     int max_replacements;  /* 0 = default 64 */
     int grid;              /* 0 = autotuned; else workgroups (must be co-resident) */
     int block;             /* 0 = default (1024) */

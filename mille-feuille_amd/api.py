@@ -201,7 +201,7 @@ def _dev_vec(v, n, device):
 
 
 def solve_options(tol=1e-10, max_iter=100000, precision="fp64", fixed_iterations=False, dynamic_precision=0,
-                  replace_delta=1e-3, max_replacements=64, grid=0, variant=0, use_x0=False, watchdog_ms=0.0,
+                  replace_delta=0.0, max_replacements=64, grid=0, variant=0, use_x0=False, watchdog_ms=0.0,
                   layout="auto", value_cache=True, transpose=False):
     o = L.SolveOptions()
     L.lib().mf_solve_options_default(ctypes.byref(o))

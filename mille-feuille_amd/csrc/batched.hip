@@ -562,7 +562,7 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     a.p32 = A->ws.p32;
     a.info = B->d_info;
     a.tol2 = o.tol * o.tol;
-    const double dl = o.replace_delta > 0 ? o.replace_delta : 1e-3;
+    const double dl = o.replace_delta > 0 ? o.replace_delta : cg_replace_delta(A);
     a.delta2 = dl * dl;
     a.maxit = o.max_iter;
     a.fixed = o.fixed_iterations;

@@ -199,4 +199,10 @@ mf_status transposed(mf_matrix *A, hipStream_t s, mf_matrix **out);
 // Tile chunks carry a 12-byte header per chunk of up to 16 entries: with few entries per
 // tile they cost more per entry than the sliced-ELL column offsets.
 inline bool sparse_tiles(const mf_matrix *A) { return A->L.tiles > 0 && A->L.nnz < 8 * A->L.tiles; }
+
+// Default residual-replacement interval of the tile-mixed CG (single-kernel and batched):
+// replace once ||r|| fell by this factor since the last replacement.  Lossless classes
+// (exactly stored values; only p is rounded, to binary32) keep the recurrence close to the
+// true residual over five decades; lossy classes over three.
+inline double cg_replace_delta(const mf_matrix *A) { return A->class_policy == MF_CLASS_LOSSLESS ? 1e-5 : 1e-3; }
 }  // namespace mf
