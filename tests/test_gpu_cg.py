@@ -207,3 +207,39 @@ def test_cg_watchdog_and_occupancy_refusal(mf, torch):
     x, info = A.cg(b, grid=8)                     # the matrix is still usable afterwards
     assert info["status"] == 0
     A.close()
+
+
+def _heavy_band(n=18000, half=100):
+    """Banded SPD matrix with 2 half + 1 entries per row (the heavy rows of nd6k or crankseg_1
+    in the reference's matrix set): symmetric off-diagonals, strictly diagonally dominant."""
+    i = np.repeat(np.arange(n), 2 * half + 1)
+    j = i + np.tile(np.arange(-half, half + 1), n)
+    keep = (j >= 0) & (j < n)
+    i, j = i[keep], j[keep]
+    v = -(0.1 + ((i + j) % 17) / 17.0) * np.exp(-np.abs(i - j) / 50.0)
+    off = i != j
+    diag = np.zeros(n)
+    np.add.at(diag, i[off], np.abs(v[off]))
+    v[~off] = diag[i[~off]] + 1.0
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, i + 1, 1)
+    return np.cumsum(rp).astype(np.int32), j.astype(np.int32), v
+
+
+def test_heavy_rows_keep_every_cu(mf, orc, torch):
+    """Rows of 201 entries: the default grid comes from the tile-chunk rule (a bandwidth-bound
+    SpMV on every CU), not from the rows-per-workgroup rule measured on stencils."""
+    rp, ci, va = _heavy_band()
+    n = rp.size - 1
+    b = np.random.default_rng(22).uniform(-1, 1, n)
+    A = mf.TiledMatrix(rp, ci, va)
+    ref = orc.cg(rp, ci, va, b, tol=1e-10)
+    for prec in ("fp64", "mixed"):
+        x, info = A.cg(b, tol=1e-10, precision=prec)
+        print(f"heavy rows {prec}: grid {info['grid']} layout {info['layout']} {info['time_ms']:.3f} ms, "
+              f"{info['iterations']} iterations (oracle {ref['iterations']})", flush=True)
+        assert info["status"] == 0
+        assert info["grid"] >= 128
+        assert abs(info["iterations"] - ref["iterations"]) <= max(1, 0.02 * ref["iterations"])
+        assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
+    A.close()
