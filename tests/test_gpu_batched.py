@@ -136,3 +136,32 @@ def test_cg_breakdown_keeps_last_finite_iterate(mf, torch, fn):
     xh = x.cpu().numpy()
     assert np.all(np.isfinite(xh)) and not xh.any()
     A.close()
+
+
+def test_batch_timeline_and_layout(mf, orc, torch):
+    """The per-system solve intervals of a launch (the kernel's clock): every system starts
+    before it ends and within the launch, the group sizes follow the rows-per-workgroup
+    bound, and the default SpMV reads the sliced-ELL streams (layout 2/3); layout 1 forces
+    the tile chunks with the same iteration counts."""
+    systems = _systems(mf, 12, 8)
+    B = mf.Batch([(r, c, v) for r, c, v, _ in systems])
+    B.set_rows_per_workgroup(1024)
+    b = B.pack([s[3] for s in systems])
+    x, infos, ms = B.cg(b, use_graph=False, tol=1e-10)
+    st, en, gs = B.timeline()
+    n = np.array([s[0].size - 1 for s in systems])
+    assert np.all(st >= 0) and np.all(en > st) and en.max() <= 1.05 * ms + 0.05
+    # the smallest power of two W with ceil(tile rows / W) * 16 <= 1024
+    want = []
+    for ni in n:
+        tr, W = (ni + 15) // 16, 1
+        while -(-tr // W) * 16 > 1024:
+            W *= 2
+        want.append(W)
+    assert list(gs) == want, (list(gs), want)
+    assert all(inf["layout"] in (2, 3) for inf in infos)
+    x1, infos1, _ = B.cg(b, use_graph=False, tol=1e-10, layout="chunks")
+    assert all(inf["layout"] == 1 and inf["status"] == 0 for inf in infos1)
+    for inf, inf1 in zip(infos, infos1):
+        assert abs(inf["iterations"] - inf1["iterations"]) <= 1
+    B.close()

@@ -243,3 +243,24 @@ def test_heavy_rows_keep_every_cu(mf, orc, torch):
         assert abs(info["iterations"] - ref["iterations"]) <= max(1, 0.02 * ref["iterations"])
         assert orc.true_relres(rp, ci, va, x.cpu().numpy(), b) <= 1.2e-10
     A.close()
+
+
+def test_phase_profile_tool(tmp_path):
+    """The per-phase timers (libmf_phase.so through phase_profile.py, in a process of its
+    own): the phases of 2D Poisson 64 x 64 add up to the instrumented kernel's time per
+    iteration, and the solve keeps the oracle's 201 iterations."""
+    import json
+    import pathlib
+    import subprocess
+    import sys
+    root = pathlib.Path(__file__).resolve().parents[1]
+    out = tmp_path / "phases.json"
+    r = subprocess.run([sys.executable, str(root / "mille-feuille_amd" / "phase_profile.py"), "--workload", "poisson64",
+                        "--precision", "fp64", "--json", str(out)], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    d = json.loads(out.read_text())
+    assert d["iterations"] == 201 and d["variant"] == 4
+    total = d["sum_of_phases_mean_us"]
+    assert 0.8 * d["kernel_us_per_iteration"] <= total <= 1.05 * d["kernel_us_per_iteration"]
+    assert all(ph["mean_us"] >= 0 for ph in d["phases"])

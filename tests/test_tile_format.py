@@ -154,3 +154,14 @@ def test_library_exports_every_declared_symbol(mf):
     lib = mf.lib()
     missing = sorted(s for s in declared if getattr(lib, s, None) is None)
     assert not missing, missing
+
+
+def test_phase_timers_only_in_the_phase_library(mf):
+    """libmf.so carries the per-phase timer entry point but no timers: it refuses (no GPU call
+    is made); libmf_phase.so, built from the same sources, exports the same symbols."""
+    import ctypes
+    buf = (ctypes.c_ulonglong * 8)()
+    assert mf.lib().mf_phase_read(buf, 1) == 10  # MF_ERR_UNSUPPORTED
+    phase = ctypes.CDLL(str(GOLDEN.parents[1] / "mille-feuille_amd" / "libmf_phase.so"))
+    for name in ("mf_phase_read", "mf_cg_solve", "mf_matrix_create", "mf_batch_timeline"):
+        assert getattr(phase, name, None) is not None, name
