@@ -368,6 +368,18 @@ def main():
     gbs_d = bytes_fp64 / (t_fp64 * 1e-3) / 1e9
 
     extra = {}
+    # partial-convergence classes (the reference's |dp| thresholds, and the scale-aware
+    # variant) against the static classes above: tile-chunk layout, variant 2
+    dyn = {}
+    for mode in (1, 2):
+        runs = [A.cg(b, tol=1e-10, precision="mixed", dynamic_precision=mode) for _ in range(3)]
+        _, inf_y = sorted(runs, key=lambda r: r[1]["time_ms"])[1]
+        dyn[f"mode{mode}"] = {"ms": inf_y["time_ms"], "iterations": inf_y["iterations"],
+                              "replacements": inf_y["replacements"], "relres_true": inf_y["relres_true"],
+                              "layout": inf_y["layout"], "variant": inf_y["variant"], "status": inf_y["status"]}
+    log(f"[bench] dynamic classes: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['iterations']} it)" for k, v in dyn.items())
+        + f"; static {t_mixed:.3f} ms")
+    extra["dynamic_classes"] = dyn
     p64 = None
     if wl == "all":
         if not args.no_bicg:
