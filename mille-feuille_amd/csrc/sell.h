@@ -251,7 +251,11 @@ __device__ __forceinline__ void sell_gather(SellUnit<S, NAR, PM> &U, int row, co
 
 // Sum of the unit's products in the arithmetic of product mode PM: class 0 in binary64;
 // classes 1-3 as binary32 products of the value and the binary32 vector, or (kPmWide64)
-// in binary64 with the value widened exactly; the products are added in binary64.
+// in binary64 with the value widened exactly; the products are added in binary64.  A
+// binary64 product is fused into the sum (fma, one rounding); a binary32 product is
+// rounded to binary32 first and then added.  The sum starts at the first product, not at
+// 0.0 + it: the two differ only in the sign of a zero sum, and the caller adds it to a sum
+// that is never -0.0.
 template <int S, bool NAR, int PM>
 __device__ __forceinline__ double sell_products(const SellUnit<S, NAR, PM> &U)
 {
@@ -260,7 +264,8 @@ __device__ __forceinline__ double sell_products(const SellUnit<S, NAR, PM> &U)
 #pragma unroll
     for (int j = 0; j < UT::E; j++) {
         if constexpr (sell_cls<S>() == 0) {
-            s += __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]) * (double)U.xg[j];
+            const double v = __hiloint2double((int)U.v.w[2 * j + 1], (int)U.v.w[2 * j]);
+            s = j == 0 ? v * (double)U.xg[j] : __builtin_fma(v, (double)U.xg[j], s);
         } else {
             float v;
             if constexpr (S == 1) {
@@ -271,8 +276,12 @@ __device__ __forceinline__ double sell_products(const SellUnit<S, NAR, PM> &U)
             } else {
                 v = (float)(signed char)(U.v.w[j >> 2] >> (8 * (j & 3)));
             }
-            if constexpr (UT::X32) s += (double)(v * U.xg[j]);
-            else s += (double)v * U.xg[j];
+            if constexpr (UT::X32) {
+                const double t = (double)(v * U.xg[j]);
+                s = j == 0 ? t : s + t;
+            } else {
+                s = j == 0 ? (double)v * U.xg[j] : __builtin_fma((double)v, U.xg[j], s);
+            }
         }
     }
     return s;
@@ -315,15 +324,22 @@ __device__ __forceinline__ void sell_prefetch(const SellStream &st, int slot, Se
 // sets rotate (the loop body is unrolled three times), so no register with a load in
 // flight is copied, and the loop has a single exit test per three steps and no other
 // branch, so the compiler keeps each load where it is issued (a branch between a load and
-// its use lets it sink the load next to the use and wait for it there).  Steps past the
-// last unit load the last unit again and add nothing.
-// The slice being summed is always q[0]: when unit i ends it, acc is added to q[0] and q
-// rotates down by one (selects, no branch).  The layout gives every empty slice between
-// two non-empty ones of a slot one zero unit, so a step ends at most one slice; leading
-// and trailing empty slices rotate outside the loop.  RPT rotations per stream restore
-// q's order.  Unit bounds are wave-uniform (scalar loads).
+// its use lets it sink the load next to the use and wait for it there).  The loop runs
+// whole triples of units; the one or two units left over run after it, outside the
+// pipeline's steady state, so no step sums a unit twice or adds nothing.
+// The slice being summed is q[rot], rot a wave-uniform index: when unit i ends the slice,
+// acc is added to q[rot] and rot advances.  q lives in a register vector, and the indexed
+// read and write are two register moves each under s_set_gpr_idx (no selects over the
+// RPT sums, which the row-per-lane loop would otherwise pay every step).  The layout gives
+// every empty slice between two non-empty ones of a slot one zero unit, so a step ends at
+// most one slice; leading empty slices advance rot before the loop.  Unit bounds are
+// wave-uniform (scalar loads).  Each slice's sum is acc = ((u0 + u1) + ...) over its
+// units in order, added to q[k] once: the same arithmetic as summing slice by slice.
 // LEAD: the first two units were loaded by sell_prefetch into *lead (whose registers the
 // pipeline then reuses: prefetch again before the next use).
+template <int N>
+using DVec = double __attribute__((ext_vector_type(N)));
+
 template <int S, int RPT, int BLK, bool NAR, int PM = kPmTiles, bool LEAD = false, bool VL = false>
 __device__ __forceinline__ void sell_stream_impl(const SellStream &st, int slot, int rowb, const double *__restrict__ x64,
                                                  const float *__restrict__ x32, double (&q)[RPT],
@@ -341,18 +357,17 @@ __device__ __forceinline__ void sell_stream_impl(const SellStream &st, int slot,
     int e[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) e[k] = __builtin_amdgcn_readfirstlane(sb[k + 1]);
-    int rot = 0;
-    auto close_slice = [&]() {
-        const double t = q[0];
+    constexpr int NV = RPT < 2 ? 2 : RPT;  // vectors of one element do not index
+    DVec<NV> qv;
 #pragma unroll
-        for (int k = 0; k + 1 < RPT; k++) {
-            q[k] = q[k + 1];
-            e[k] = e[k + 1];
-        }
-        q[RPT - 1] = t;
+    for (int k = 0; k < NV; k++) qv[k] = k < RPT ? q[k] : 0.0;
+    int rot = 0;
+    auto advance = [&]() {
+#pragma unroll
+        for (int k = 0; k + 1 < RPT; k++) e[k] = e[k + 1];
         rot++;
     };
-    while (rot < RPT - 1 && e[0] <= u0) close_slice();  // leading empty slices
+    while (rot < RPT - 1 && e[0] <= u0) advance();  // leading empty slices
     using UT = SellUnit<S, NAR, PM>;
     UT l0, l1, S2;
     UT &S0 = LEAD ? lead->u0 : l0;
@@ -364,6 +379,21 @@ __device__ __forceinline__ void sell_stream_impl(const SellStream &st, int slot,
         sell_load<S, NAR, PM, VL>(st, S1, min(i + 1, last), lane, ilane, uw0);
     }
     sell_gather<S, NAR, PM>(S0, rowb, X);
+    // unit i's sum into acc; when it ends the slice, acc into q[rot] (the index clamped
+    // so that no register outside the vector is ever named).  readfirstlane states that
+    // rot is wave-uniform, which the compiler does not prove here: it would keep rot in a
+    // vector register and index in a loop over lanes.
+    auto sum_unit = [&](const UT &a) {
+        acc += sell_products<S, NAR, PM>(a);
+        const bool cl = i + 1 == e[0];
+        const int r = min(rot, RPT - 1);
+        qv[r] = qv[r] + (cl ? acc : 0.0);
+        acc = cl ? 0.0 : acc;
+#pragma unroll
+        for (int k = 0; k + 1 < RPT; k++) e[k] = cl ? e[k + 1] : e[k];
+        rot = __builtin_amdgcn_readfirstlane(rot + (cl ? 1 : 0));
+        i++;
+    };
     // issue order within a step: unit i+2's words, unit i+1's gathers, then unit i's sum;
     // the loads complete in issue order (vmcnt), so waiting for unit i+1's column words
     // leaves unit i's gathers in flight, and waiting for those leaves this step's loads
@@ -372,31 +402,29 @@ __device__ __forceinline__ void sell_stream_impl(const SellStream &st, int slot,
         __builtin_amdgcn_sched_barrier(0);
         sell_gather<S, NAR, PM>(nb1, rowb, X);
         __builtin_amdgcn_sched_barrier(0);
-        const double pr = sell_products<S, NAR, PM>(a);
-        acc += i <= last ? pr : 0.0;
-        const bool cl = i + 1 == e[0];  // unit i ends the current slice
-        const double q0 = q[0] + acc;
-#pragma unroll
-        for (int k = 0; k + 1 < RPT; k++) {
-            q[k] = cl ? q[k + 1] : q[k];
-            e[k] = cl ? e[k + 1] : e[k];
-        }
-        q[RPT - 1] = cl ? q0 : q[RPT - 1];
-        acc = cl ? 0.0 : acc;
-        rot += cl ? 1 : 0;
-        i++;
+        sum_unit(a);
     };
     // scheduling barriers keep each step's loads in issue order, so every wait is for a
     // load issued a step earlier
-    do {
+    while (i + 2 <= last) {
         step(S0, S1, S2);
         __builtin_amdgcn_sched_barrier(0);
         step(S1, S2, S0);
         __builtin_amdgcn_sched_barrier(0);
         step(S2, S0, S1);
         __builtin_amdgcn_sched_barrier(0);
-    } while (i <= last);
-    while (rot < RPT) close_slice();  // trailing empty slices
+    }
+    // one or two units left: unit i is gathered in S0, unit i+1's words are in S1
+    if (i < last) {
+        sell_gather<S, NAR, PM>(S1, rowb, X);
+        __builtin_amdgcn_sched_barrier(0);
+        sum_unit(S0);
+        sum_unit(S1);
+    } else if (i == last) {
+        sum_unit(S0);
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; k++) q[k] = qv[k];
 }
 
 // One stream's contribution; VL: values from the workgroup's LDS copy.
