@@ -363,7 +363,8 @@ mf_status mf_batch_row_offsets(const mf_batch *B, int64_t *offsets /* count+1 */
  * use_graph != 0 the launch is captured once into a hipGraph and replayed.  The SpMV
  * reads the sliced-ELL streams of the member ranges (per-system info layout 2/3) unless
  * opt->layout = 1 or a system's columns lie beyond the streams' 16-bit offsets (tile
- * chunks, layout 1). */
+ * chunks, layout 1).  Every launch starts from x = 0; opt->use_x0, opt->transpose and
+ * opt->dynamic_precision are refused (MF_ERR_UNSUPPORTED). */
 mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double *d_x, const mf_solve_options *opt,
                               int use_graph, mf_solve_info *per_system, double *time_ms, void *stream);
 /* Bound on the rows one workgroup of a system holds (1024, 2048, 4096 or 8192; default

@@ -503,6 +503,9 @@ extern "C" mf_status mf_cg_solve_batched(mf_batch *B, const double *d_b, double 
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    // every launch starts from x = 0 (the reset clears x with the publish buffers), and
+    // the batched core has neither the transposed streams nor the dynamic classes
+    if (o.use_x0 || o.transpose || o.dynamic_precision) return MF_ERR_UNSUPPORTED;
     const bool mixed = o.precision == MF_PREC_MIXED && A->L.has_lowp;
     const int rpt = B->rows_per_wg / 1024;
     BatchKernel k = mixed ? pick_batch<true>(rpt, B->unroll) : pick_batch<false>(rpt, B->unroll);

@@ -112,6 +112,22 @@ def test_graph_replay_follows_changed_options(mf, orc, torch):
     B.close()
 
 
+@pytest.mark.parametrize("opt", [{"use_x0": True}, {"transpose": True}, {"dynamic_precision": 1}])
+def test_batched_refuses_options_it_cannot_honour(mf, torch, opt):
+    """Every batched launch starts from x = 0 and has no transposed or dynamic-class path:
+    asking for one is an error, not a silently different solve."""
+    systems = _systems(mf, 3, 11)
+    B = mf.Batch([(r, c, v) for r, c, v, _ in systems], class_policy="lossless")
+    b = B.pack([s[3] for s in systems])
+    for graph in (False, True):
+        with pytest.raises(mf.MfError) as e:
+            B.cg(b, use_graph=graph, tol=1e-10, precision="mixed", **opt)
+        assert "UNSUPPORTED" in str(e.value)
+    x, infos, _ = B.cg(b, use_graph=False, tol=1e-10, precision="mixed")  # still usable
+    assert all(i["status"] == 0 for i in infos)
+    B.close()
+
+
 @pytest.mark.parametrize("fn", ["mf_cg_solve_multikernel", "mf_cg_solve"])
 def test_cg_breakdown_keeps_last_finite_iterate(mf, torch, fn):
     """p'Ap = 0 at the first step (blocks [[0, 1], [1, 0]], b = e_1): MF_BREAKDOWN, and
