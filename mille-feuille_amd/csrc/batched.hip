@@ -89,6 +89,7 @@ __device__ void batch_worker(const BatchArgs &a, int pool, int W, int member, in
     gsy.gs.ctl = a.ctl;
     gsy.gs.timeout = a.timeout;
     gsy.gs.G = W;
+    gsy.gs.a2a = W <= kGridA2aMax;
     gsy.gs.acq = SELL ? 0 : 1;  // tile-chunk gathers are plain loads, sliced-ELL gathers agent-scope
     gsy.wg = member;
     gsy.epoch = 0;

@@ -51,7 +51,12 @@ struct GridSync {
     // another workgroup published is itself an agent-scope load (the sliced-ELL gathers),
     // so the L2 keeps the matrix stream from one iteration to the next
     int acq;
+    // 1: every member polls all member records and combines them itself (one hop, no
+    // broadcast); 0: member 0 gathers and broadcasts (grid_sum)
+    int a2a;
 };
+// Largest group that reduces with every member combining all records (GridSync::a2a).
+constexpr int kGridA2aMax = 64;
 
 __device__ __forceinline__ void wait_own_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
 
@@ -340,9 +345,12 @@ __device__ __forceinline__ bool poll_record(const GridSync &gs, const unsigned l
 // poll only the broadcast record with K lanes.  Polling every member record from every
 // workgroup put hundreds of thousands of agent-scope loads on a few cache lines and
 // cost about 10 us per reduction at 256 workgroups; two hops with few pollers cost far
-// less.  A failed wait (watchdog, or the abort flag set by another workgroup) travels in
-// the broadcast tag, so every member leaves at the same sync without a further read of
-// the flag.  Returns false when the group aborts.
+// less.  Small groups (gs.a2a, at most kGridA2aMax members) skip the broadcast: every
+// member polls all records and combines them in member 0's fixed order, so every member
+// computes the same bits.  A failed wait (watchdog, or the abort flag set by another
+// workgroup) travels in the broadcast tag, so every member leaves at the same sync without
+// a further read of the flag; without the broadcast each member's own polls see the abort
+// flag.  Returns false when the group aborts.
 template <int K>
 __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], unsigned &epoch, bool visibility, int wg,
                          double *scratch)
@@ -386,7 +394,7 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
     phase_record(wg, ph0);
     // keep the compiler from moving the record stores behind this thread's polling
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (wg == 0) {
+    if (wg == 0 || gs.a2a) {
         double acc[K];
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] = 0.0;
@@ -408,7 +416,7 @@ __device__ __forceinline__ bool grid_sum(const GridSync &gs, double (&v)[K], uns
         if (threadIdx.x < 64) {
             const double t = block_combine<K>(scratch, nwv);
             if (klead) {
-                put_record(rec_at(gs, epoch, kl, G), t, s_fail ? (epoch | kAbortTag) : epoch);
+                if (!gs.a2a) put_record(rec_at(gs, epoch, kl, G), t, s_fail ? (epoch | kAbortTag) : epoch);
                 s_bc[kl] = t;
             }
         }

@@ -480,6 +480,7 @@ mf_status bicg_launch(mf_matrix *A, const double *d_b, double *d_x, const mf_sol
     a.gs.ctl = A->ws.ctl;
     a.gs.timeout = watchdog_cycles(o->watchdog_ms);
     a.gs.G = G;
+    a.gs.a2a = G <= mfk::kGridA2aMax;
     a.gs.acq = sell ? 0 : 1;  // sliced-ELL gathers are agent-scope loads
     a.b = d_b;
     a.x = d_x;
