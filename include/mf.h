@@ -204,7 +204,8 @@ typedef struct mf_solve_options {
     int precision;         /* MF_PREC_FP64 or MF_PREC_MIXED */
     int fixed_iterations;  /* reference mode: run exactly max_iter iterations */
     int dynamic_precision; /* 0 off; 1 partial-convergence demotion (|dp| relative to ||p||_inf);
-                              2 the reference's absolute |dp| thresholds */
+                              2 the reference's absolute |dp| thresholds.  Single-kernel CG
+                              only: the other solvers refuse it (MF_ERR_UNSUPPORTED) */
     double replace_delta;  /* mixed: replace r by b - A x once ||r|| < delta * ||r|| at the last
                               replacement (0 = the solver's default: for the single-kernel and
                               batched CG 1e-5 under lossless classes, else 1e-3; 1e-3 for

@@ -549,6 +549,7 @@ extern "C" mf_status mf_bicgstab_solve(mf_matrix *A, const double *d_b, double *
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    if (o.dynamic_precision) return MF_ERR_UNSUPPORTED;  // partial-convergence classes: CG only
     if (o.transpose) {
         // A^T x = b: the same solve on the transposed tiled matrix
         mf_matrix *T = nullptr;

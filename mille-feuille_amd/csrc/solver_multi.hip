@@ -438,6 +438,7 @@ extern "C" mf_status mf_cg_solve_multikernel(mf_matrix *A, const double *d_b, do
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    if (o.dynamic_precision) return MF_ERR_UNSUPPORTED;  // single-kernel CG only
     if (o.transpose) {
         mf_status st = transposed(A, (hipStream_t)stream, &A);
         if (st != MF_SUCCESS) return st;
@@ -454,6 +455,7 @@ extern "C" mf_status mf_bicgstab_solve_multikernel(mf_matrix *A, const double *d
     mf_solve_options o;
     mf_solve_options_default(&o);
     if (opt) o = *opt;
+    if (o.dynamic_precision) return MF_ERR_UNSUPPORTED;  // single-kernel CG only
     if (o.transpose) {
         mf_status st = transposed(A, (hipStream_t)stream, &A);
         if (st != MF_SUCCESS) return st;

@@ -70,6 +70,21 @@ def test_multikernel_cg_and_bicgstab(mf, orc, torch, prec):
     A2.close()
 
 
+def test_dynamic_classes_only_in_single_kernel_cg(mf, torch):
+    """Partial-convergence classes exist in the single-kernel CG; BiCGSTAB and the
+    multi-kernel solvers refuse the option instead of ignoring it."""
+    rp, ci, va = mf.workloads.poisson2d(24)
+    b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
+    A = mf.TiledMatrix(rp, ci, va)
+    for fn in (A.bicgstab, A.cg_multikernel, A.bicgstab_multikernel):
+        with pytest.raises(mf.MfError) as e:
+            fn(b, tol=1e-10, precision="mixed", dynamic_precision=1)
+        assert "UNSUPPORTED" in str(e.value)
+    x, info = A.cg(b, tol=1e-10, precision="mixed", dynamic_precision=1)
+    assert info["status"] == 0
+    A.close()
+
+
 def test_value_update_and_pattern_outer(mf, orc, torch):
     rp, ci, va = mf.workloads.poisson2d(20)
     b, _ = mf.workloads.rhs_from_solution(rp, ci, va, 1)
